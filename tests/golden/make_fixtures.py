@@ -1,0 +1,53 @@
+"""Writes the small fixtures under tests/golden/ from the reference's own test data.
+
+Run in the build container (where the read-only reference checkout exists):
+    python tests/golden/make_fixtures.py /root/reference
+
+Only data files are read (no reference code is imported or run):
+  test-data/protein-query.fasta                         (30,000 single-line records)
+  regression_test/test_regression_protein_good_result   (30,000 golden top scores)
+  regression_test/test_regression_dna_good_result       (30,000 golden top scores)
+
+Output:
+  protein_query_golden.tsv : first N records as "<query>\t<golden score>"
+  dna_golden_scores.txt    : the 30,000 DNA golden scores (inputs are not shipped)
+The protein reference sequences and all DNA inputs are missing from the
+reference checkout (.MISSING_LARGE_BLOBS), so the goldens can only bound scores.
+"""
+import os
+import sys
+
+N_PROTEIN = 3000
+
+
+def read_single_line_fasta(path):
+    """Same reader convention as main.cpp:22-36: skip lines starting with '>',
+    every other line is one sequence."""
+    seqs = []
+    with open(path) as fh:
+        for line in fh:
+            line = line.rstrip("\n")
+            if line.startswith(">"):
+                continue
+            seqs.append(line)
+    return seqs
+
+
+def main(ref_root):
+    here = os.path.dirname(os.path.abspath(__file__))
+    queries = read_single_line_fasta(os.path.join(ref_root, "test-data", "protein-query.fasta"))
+    with open(os.path.join(ref_root, "regression_test", "test_regression_protein_good_result")) as fh:
+        golden = [int(x.split()[0]) for x in fh if x.strip()]
+    assert len(queries) == len(golden) == 30000, (len(queries), len(golden))
+    with open(os.path.join(here, "protein_query_golden.tsv"), "w") as out:
+        for q, g in zip(queries[:N_PROTEIN], golden[:N_PROTEIN]):
+            out.write("%s\t%d\n" % (q, g))
+    with open(os.path.join(ref_root, "regression_test", "test_regression_dna_good_result")) as fh:
+        dna = [int(x.split()[0]) for x in fh if x.strip()]
+    with open(os.path.join(here, "dna_golden_scores.txt"), "w") as out:
+        out.write("\n".join(str(v) for v in dna) + "\n")
+    print("wrote %d protein and %d DNA golden rows" % (N_PROTEIN, len(dna)))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,352 @@
+"""Pins the CPU oracle (oracle/gbsw_oracle.c) before anything is compared to it.
+
+Each check is independent of the oracle's own closed form:
+  * a literal per-thread model of the reference kernels (tests/kernel_model.py)
+    on random tiny inputs with small alphabets (forcing ties), several scoring
+    schemes and several CUDA block sizes (partial warps, two-level trees);
+  * exhaustive enumeration of every local alignment on tiny inputs (score);
+  * a textbook Gotoh with -inf boundaries (score; the reference's boundary
+    quirks must not change H when gap scores are <= 0);
+  * closed forms (identical sequences, disjoint alphabets) and invariants
+    (the reverse-pass maximum equals the forward score);
+  * the hand-derived tie fixture worked out from kernel.cpp;
+  * the reference's protein regression goldens, as an upper bound.
+"""
+import itertools
+import os
+import random
+
+import numpy as np
+import pytest
+
+import kernel_model
+import oracle_lib as ol
+from gbsw.scoring import AA_ORDER, BLOSUM62, encode_aa
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+DNA_SCHEMES = [(1, -3, -3, -1), (3, -3, -6, -1), (2, -1, -2, -2), (1, -1, -1, -1), (5, -4, -10, 0)]
+
+
+def dna_fn(match, mismatch):
+    return lambda a, b: match if a == b else mismatch
+
+
+def aa_fn(a, b):
+    return BLOSUM62[encode_aa(ord(a)) * 24 + encode_aa(ord(b))]
+
+
+def rand_seq(rng, n, alpha):
+    return "".join(rng.choice(alpha) for _ in range(n))
+
+
+# ---------------------------------------------------------------- fixtures
+
+
+def test_survey_tie_fixture():
+    """Full tie in row 9 of the forward pass: column t=8 has the smaller
+    bit-reversed tree priority than t=3 (kernel.cpp:59-80)."""
+    sc = ol.dna_scoring(1, -3, -3, -1)
+    assert ol.align("GGGGGACGTGGGG", "ACGTTACGT", sc) == (4, 5, 9, 5, 9)
+    m = kernel_model.align("GGGGGACGTGGGG", "ACGTTACGT", dna_fn(1, -3), -3, -1)
+    assert m == (4, 5, 9, 5, 9)
+
+
+def test_orientation_counter_example():
+    """With the query longer than the ref, the query is the row sequence, so the
+    winner is the smaller row of the query (kernel.cpp:248-253, 381-386)."""
+    sc = ol.dna_scoring(1, -3, -3, -1)
+    got = ol.align("ACGT", "ACGTTACGT", sc)
+    assert got == kernel_model.align("ACGT", "ACGTTACGT", dna_fn(1, -3), -3, -1)
+    assert got == (4, 0, 4, 0, 4)
+
+
+def test_prio_order():
+    """First lanes of the 32-wide tree in priority order: 0,16,8,24,4,20,..."""
+    order = sorted(range(32), key=ol.prio)
+    assert order[:8] == [0, 16, 8, 24, 4, 20, 12, 28]
+    assert ol.prio(32) == 16 * 32  # warp 1 = bitrev5(1) = 16
+    # warp index is the major key: all of warp 0 first, then warp 16 (t=512)
+    full = sorted(range(1024), key=ol.prio)
+    assert full[:3] == [0, 16, 8] and full[32:35] == [512, 528, 520]
+    assert ol.prio(1024) > max(ol.prio(t) for t in range(1024))
+
+
+# ---------------------------------------------------- literal kernel model
+
+
+def _model_cases(seed, n, alphabets, kind="dna"):
+    rng = random.Random(seed)
+    cases = []
+    for _ in range(n):
+        alpha = rng.choice(alphabets)
+        la, lb = rng.randint(1, 45), rng.randint(1, 45)
+        a, b = rand_seq(rng, la, alpha), rand_seq(rng, lb, alpha)
+        if kind == "dna":
+            scheme = rng.choice(DNA_SCHEMES)
+        else:
+            scheme = None
+        mn = min(la, lb)
+        T = rng.choice([mn, mn + rng.randint(0, 40), max(mn, 33), max(mn, 64) + rng.randint(0, 3)])
+        cases.append((a, b, scheme, T))
+    return cases
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_matches_kernel_model_dna(seed):
+    agree = 0
+    for a, b, scheme, T in _model_cases(seed, 160, ["AC", "ACG", "ACGT", "AAC"]):
+        m, mm, go, ge = scheme
+        sc = ol.dna_scoring(m, mm, go, ge)
+        got = ol.align(a, b, sc)
+        s, ea, eb = kernel_model.forward(a, b, T, dna_fn(m, mm), go, ge)
+        if s == 0:
+            assert got == (0, 0, 0, 0, 0)
+            continue
+        ba, bb = kernel_model.reverse(a, b, ea, eb, max(1, min(ea, eb)) + (T % 7), dna_fn(m, mm), go, ge)
+        assert got == (s, ba, ea, bb, eb), (a, b, scheme, T)
+        agree += 1
+    assert agree > 100
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_oracle_matches_kernel_model_protein(seed):
+    sc = ol.aa_scoring(BLOSUM62, -6, -1)
+    agree = 0
+    for a, b, _, T in _model_cases(seed, 100, ["AG", "ARN", "WYVX", "AEQKD*B"], kind="aa"):
+        got = ol.align(a, b, sc)
+        s, ea, eb = kernel_model.forward(a, b, T, aa_fn, -6, -1)
+        if s == 0:
+            assert got == (0, 0, 0, 0, 0)
+            continue
+        ba, bb = kernel_model.reverse(a, b, ea, eb, max(1, min(ea, eb)), aa_fn, -6, -1)
+        assert got == (s, ba, ea, bb, eb), (a, b, T)
+        agree += 1
+    assert agree > 60
+
+
+def _tie_heavy_cases(seed, n):
+    """Column sequence = copies of one motif separated by filler, row sequence
+    = the motif once inside filler: the forward maximum is then reached in one
+    row at several columns (same-row ties, decided by tree priority), and many
+    columns of the reverse pass reach it too.  Half the cases use equal lengths
+    or equal end coordinates to exercise the orientation ties."""
+    rng = random.Random(seed)
+    cases = []
+    for k in range(n):
+        motif = rand_seq(rng, rng.randint(2, 6), "ACGT")
+        fill = rng.choice(["G", "T", "GT"])
+        reps = rng.randint(2, 12)
+        cols = "".join(motif + rand_seq(rng, rng.randint(0, 5), fill) for _ in range(reps))
+        rows = rand_seq(rng, rng.randint(0, 30), fill) + motif + rand_seq(rng, rng.randint(0, 30), fill)
+        if len(rows) <= len(cols):
+            rows += rand_seq(rng, len(cols) - len(rows) + rng.randint(0, 4), fill)
+        if k % 4 == 1:  # equal lengths: ref is the row sequence
+            cols = cols[:len(rows)] if len(cols) > len(rows) else cols + rand_seq(rng, len(rows) - len(cols), fill)
+        ref, qry = (rows, cols) if rng.random() < 0.5 else (cols, rows)
+        if k % 4 == 3:  # identical prefixes make ref_end == query_end likely
+            qry = ref[:rng.randint(1, len(ref))] + rand_seq(rng, rng.randint(0, 4), fill)
+        cases.append((ref, qry))
+    return cases
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_oracle_matches_kernel_model_ties(seed):
+    sc = ol.dna_scoring(1, -3, -3, -1)
+    for ref, qry in _tie_heavy_cases(seed, 120):
+        T = max(1, min(len(ref), len(qry)))
+        T += random.Random(len(ref) * 131 + len(qry)).choice([0, 0, 5, 40])
+        exp = kernel_model.align(ref, qry, dna_fn(1, -3), -3, -1, T=T)
+        got = ol.align(ref, qry, sc)
+        assert got == (exp if exp[0] else (0, 0, 0, 0, 0)), (ref, qry, T)
+
+
+def test_oracle_matches_kernel_model_long_rows():
+    """Rows far longer than the block (retired-lane spill path, diag >= maxSize)."""
+    rng = random.Random(99)
+    for _ in range(25):
+        a = rand_seq(rng, rng.randint(60, 120), "ACG")
+        b = rand_seq(rng, rng.randint(3, 40), "ACG")
+        if rng.random() < 0.5:
+            a, b = b, a
+        sc = ol.dna_scoring(2, -3, -5, -2)
+        got = ol.align(a, b, sc)
+        exp = kernel_model.align(a, b, dna_fn(2, -3), -5, -2)
+        if exp[0] == 0:
+            assert got == (0, 0, 0, 0, 0)
+        else:
+            assert got == exp
+
+
+# ------------------------------------------------------------ brute force
+
+
+def _brute_force_score(R, C, s, go, ge):
+    """Maximum over every local alignment (path enumeration), affine gaps:
+    a gap run of length L scores go + (L-1)*ge."""
+    best = 0
+    nr, nc = len(R), len(C)
+
+    def walk(i, j, acc, state):
+        nonlocal best
+        best = max(best, acc)
+        if i < nr and j < nc:
+            walk(i + 1, j + 1, acc + s(R[i], C[j]), 0)
+        if i < nr:
+            walk(i + 1, j, acc + (ge if state == 1 else go), 1)
+        if j < nc:
+            walk(i, j + 1, acc + (ge if state == 2 else go), 2)
+
+    for i0 in range(nr):
+        for j0 in range(nc):
+            # an alignment starts with an aligned pair (a leading gap never helps)
+            walk(i0 + 1, j0 + 1, s(R[i0], C[j0]), 0)
+    return best
+
+
+def test_score_matches_brute_force():
+    rng = random.Random(5)
+    for _ in range(60):
+        m, mm, go, ge = rng.choice(DNA_SCHEMES)
+        a = rand_seq(rng, rng.randint(1, 5), "ACG")
+        b = rand_seq(rng, rng.randint(1, 5), "ACG")
+        got = ol.align(a, b, ol.dna_scoring(m, mm, go, ge))[0]
+        assert got == _brute_force_score(a, b, dna_fn(m, mm), go, ge), (a, b, (m, mm, go, ge))
+    sc = ol.aa_scoring(BLOSUM62, -6, -1)
+    for _ in range(30):
+        a = rand_seq(rng, rng.randint(1, 5), "WYFHA")
+        b = rand_seq(rng, rng.randint(1, 5), "WYFHA")
+        assert ol.align(a, b, sc)[0] == _brute_force_score(a, b, aa_fn, -6, -1)
+
+
+def _textbook_gotoh(R, C, s, go, ge):
+    NEG = -(10 ** 9)
+    nr, nc = len(R), len(C)
+    H = [[0] * (nc + 1) for _ in range(nr + 1)]
+    E = [[NEG] * (nc + 1) for _ in range(nr + 1)]
+    F = [[NEG] * (nc + 1) for _ in range(nr + 1)]
+    best = 0
+    for i in range(1, nr + 1):
+        for j in range(1, nc + 1):
+            E[i][j] = max(E[i][j - 1] + ge, H[i][j - 1] + go)
+            F[i][j] = max(F[i - 1][j] + ge, H[i - 1][j] + go)
+            H[i][j] = max(0, H[i - 1][j - 1] + s(R[i - 1], C[j - 1]), E[i][j], F[i][j])
+            best = max(best, H[i][j])
+    return best
+
+
+def test_score_matches_textbook_gotoh():
+    rng = random.Random(6)
+    for _ in range(150):
+        m, mm, go, ge = rng.choice(DNA_SCHEMES)
+        a = rand_seq(rng, rng.randint(1, 60), "ACGT")
+        b = rand_seq(rng, rng.randint(1, 60), "ACGT")
+        assert ol.align(a, b, ol.dna_scoring(m, mm, go, ge))[0] == _textbook_gotoh(a, b, dna_fn(m, mm), go, ge)
+
+
+# ------------------------------------------------------ closed forms, laws
+
+
+def test_identical_sequences():
+    rng = random.Random(7)
+    sc = ol.dna_scoring(3, -3, -6, -1)
+    for n in [1, 2, 17, 64, 150, 333]:
+        a = rand_seq(rng, n, "ACGT")
+        assert ol.align(a, a, sc) == (3 * n, 0, n, 0, n)
+    sca = ol.aa_scoring(BLOSUM62, -6, -1)
+    a = rand_seq(rng, 97, AA_ORDER[:20])
+    self_score = sum(BLOSUM62[encode_aa(ord(c)) * 25] for c in a)
+    assert ol.align(a, a, sca) == (self_score, 0, 97, 0, 97)
+
+
+def test_disjoint_alphabets_score_zero():
+    sc = ol.dna_scoring(3, -3, -6, -1)
+    assert ol.align("AAAA", "CCCCCC", sc) == (0, 0, 0, 0, 0)
+    assert ol.align("", "ACGT", sc) == (0, 0, 0, 0, 0)
+    assert ol.align("ACGT", "", sc) == (0, 0, 0, 0, 0)
+
+
+def test_reverse_max_equals_forward_score():
+    rng = random.Random(8)
+    for _ in range(100):
+        m, mm, go, ge = rng.choice(DNA_SCHEMES)
+        sc = ol.dna_scoring(m, mm, go, ge)
+        a = rand_seq(rng, rng.randint(1, 80), "ACG")
+        b = rand_seq(rng, rng.randint(1, 80), "ACG")
+        s, ba, ea, bb, eb = ol.align(a, b, sc)
+        if s:
+            assert ol.reverse_max(a, b, ea, eb, sc) == s
+            assert 0 <= ba < ea <= len(a) and 0 <= bb < eb <= len(b)
+
+
+def test_swap_symmetry_of_score():
+    """The score is symmetric in (ref, query) for DNA and for the symmetric
+    BLOSUM62; coordinates may differ because orientation changes."""
+    rng = random.Random(9)
+    sca = ol.aa_scoring(BLOSUM62, -6, -1)
+    for _ in range(50):
+        a = rand_seq(rng, rng.randint(1, 70), "ARNDCQEGHILKMFPSTWYV")
+        b = rand_seq(rng, rng.randint(1, 70), "ARNDCQEGHILKMFPSTWYV")
+        assert ol.align(a, b, sca)[0] == ol.align(b, a, sca)[0]
+
+
+def test_invalid_protein_residue_rejected():
+    sca = ol.aa_scoring(BLOSUM62, -6, -1)
+    with pytest.raises(ValueError):
+        ol.align("ACDa", "ACD", sca)  # 'a' = 97 >= 91
+    assert ol.encode_aa(ord("J")) == 0 and ol.encode_aa(ord("*")) == 23
+
+
+def test_batch_driver_matches_single_calls():
+    rng = random.Random(10)
+    sc = ol.dna_scoring(3, -3, -6, -1)
+    refs = [rand_seq(rng, rng.randint(0, 90), "ACGT") for _ in range(300)]
+    qrys = [rand_seq(rng, rng.randint(0, 60), "ACGT") for _ in range(300)]
+    rb = np.frombuffer("".join(refs).encode(), dtype=np.uint8)
+    qb = np.frombuffer("".join(qrys).encode(), dtype=np.uint8)
+    ro = np.concatenate([[0], np.cumsum([len(x) for x in refs])])
+    qo = np.concatenate([[0], np.cumsum([len(x) for x in qrys])])
+    out = ol.align_batch(rb, ro, qb, qo, sc, nthreads=4)
+    for k in range(300):
+        assert tuple(int(v) for v in out[k]) == ol.align(refs[k], qrys[k], sc)
+
+
+# ------------------------------------------------ reference golden vectors
+
+
+def test_protein_goldens_are_bounded_by_oracle():
+    """The reference's protein regression scores (regression_test/
+    test_regression_protein_good_result) come from pairing each query with a
+    reference sequence that is not shipped.  No alignment of a query can score
+    more than aligning it with its per-residue best partner, which the oracle
+    scores as the sum of BLOSUM62 row maxima; every golden must respect it."""
+    path = os.path.join(HERE, "golden", "protein_query_golden.tsv")
+    sca = ol.aa_scoring(BLOSUM62, -6, -1)
+    best_partner = {}
+    for c in AA_ORDER:
+        row = BLOSUM62[AA_ORDER.index(c) * 24:(AA_ORDER.index(c) + 1) * 24]
+        best_partner[c] = AA_ORDER[int(np.argmax(row))]
+    n = 0
+    with open(path) as fh:
+        for k, line in enumerate(fh):
+            if k % 10:
+                continue
+            q, g = line.rstrip("\n").split("\t")
+            partner = "".join(best_partner[c] if c in best_partner else "A" for c in q)
+            s = ol.align(partner, q, sca)[0]
+            assert int(g) <= s
+            n += 1
+    assert n == 300
+
+
+def test_orientation_tie_fixtures():
+    """Committed fixtures where a wrong tie in the choice of the row sequence
+    (equal lengths forward, equal ends reverse) changes the coordinates."""
+    import json
+    with open(os.path.join(HERE, "golden", "tie_fixtures.json")) as fh:
+        fx = json.load(fh)
+    m, mm, go, ge = fx["scoring"]
+    sc = ol.dna_scoring(m, mm, go, ge)
+    for ref, qry, exp in fx["cases"]:
+        assert ol.align(ref, qry, sc) == tuple(exp)
+        assert kernel_model.align(ref, qry, dna_fn(m, mm), go, ge) == tuple(exp)
