@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched Smith-Waterman hot path on one MI355X.
+
+A "step" is one full alignment (forward + reverse, all five outputs) of the
+1,000,000-pair uneven-length DNA mix (config 2) already resident in HBM, through
+the C ABI (gbsw_batch_align).  The headline value is GCUPS of the batched form:
+sum(|ref|*|query|) per step / wall seconds (reference convention: forward cells
+counted, forward + reverse timed).  The same batch is also timed in the plain
+form (one forward and one reverse launch, input order, one geometry), and the
+CPU oracle is timed on a bounded sample of the same workload.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "gpu-bsw_amd"))
+
+import numpy as np  # noqa: E402
+
+import gbsw  # noqa: E402
+from gbsw import synth  # noqa: E402
+
+# MI355X: 256 CUs, 2.4 GHz peak engine clock, 64 lanes per CU per clock for
+# 32-bit integer VALU; packed int16 doubles the cell rate.  12 integer ops per
+# cell update (E: add+max, F: add+max, shared H+open, substitution, diagonal add,
+# max3+max0, best tracking cmp+max+select).
+MI355X_CU, MI355X_GHZ, OPS_PER_CELL = 256, 2.4, 12
+PEAK_INT_TOPS_PACKED = MI355X_CU * 64 * MI355X_GHZ * 2 / 1000.0  # 78.6 TOP/s (int16x2)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, sample_pairs, threads):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as ol
+    sub = w.subset(np.arange(min(sample_pairs, w.n)))
+    sc = ol.dna_scoring(*w.scoring)
+    nt = ol.load().gbsw_oracle_threads(threads)
+    t0 = time.perf_counter()
+    ol.align_batch(sub.ref, sub.ref_off, sub.query, sub.query_off, sc, nthreads=nt)
+    dt = time.perf_counter() - t0
+    return {"value": round(sub.cells() / dt / 1e9, 4), "unit": "GCUPS", "cores": int(nt), "kind": "port",
+            "sample": "first %d pairs of the workload (%.3g cells), oracle fwd+rev, %.1f s" % (sub.n, sub.cells(), dt)}
+
+
+def time_mode(batch, scoring, mode, warmup, steps, cells):
+    for _ in range(warmup):
+        batch.align(scoring, mode)
+    kernel_ms, device_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.align(scoring, mode)
+        st = batch.stats()
+        kernel_ms.append(st["main_kernel_ms"])
+        device_ms.append(st["device_ms"])
+    wall = time.perf_counter() - t0
+    return {"wall_s": wall, "ms_per_step": 1000.0 * wall / steps, "gcups": cells * steps / wall / 1e9,
+            "kernel_ms": float(np.mean(kernel_ms)), "device_ms": float(np.mean(device_ms)),
+            "device_gcups": cells / (np.mean(device_ms) / 1000.0) / 1e9, "launches": st["launches"],
+            "buckets": st["buckets"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=1_000_000, help="pairs of the DNA mix (default: full config)")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-pairs", type=int, default=200_000)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (pairs are independent; "
+              "sharding by cell count is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    t = time.perf_counter()
+    w = synth.batched_dna(args.pairs)
+    cells = w.cells()
+    log("workload %s: %d pairs, %.4g cells, generated in %.1f s" % (w.name, w.n, cells, time.perf_counter() - t))
+
+    ctx = gbsw.Context(0)
+    batch = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
+    scoring = gbsw.dna_scoring(*w.scoring)
+
+    batched = time_mode(batch, scoring, "batched", args.warmup, args.steps, cells)
+    log("batched: %.1f GCUPS wall, %.1f GCUPS device, %.2f ms/step, %d launches" %
+        (batched["gcups"], batched["device_gcups"], batched["ms_per_step"], batched["launches"]))
+    res_b = batch.results()
+    plain = None
+    if not args.no_plain:
+        plain = time_mode(batch, scoring, "plain", 1, args.plain_steps, cells)
+        log("plain:   %.1f GCUPS wall, %.1f GCUPS device, %.2f ms/step" %
+            (plain["gcups"], plain["device_gcups"], plain["ms_per_step"]))
+        res_p = batch.results()
+        same = all(np.array_equal(res_b[f], res_p[f]) for f in gbsw.FIELDS)
+        if not same:
+            log("ERROR: plain and batched results differ")
+            sys.exit(1)
+
+    # dominant kernel: all launches of one batched step (each bucket's persistent
+    # kernel runs forward+reverse); algorithmic work = 12 ops x forward cells
+    achieved = OPS_PER_CELL * cells / (batched["kernel_ms"] / 1000.0) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_INT_TOPS_PACKED, 1),
+                "unit": "TOP/s", "frac": round(achieved / PEAK_INT_TOPS_PACKED, 4), "traffic": None,
+                "ops_per_cell": OPS_PER_CELL, "kernel_ms": round(batched["kernel_ms"], 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, args.cpu_pairs, args.cpu_threads)
+        log("cpu oracle: %s GCUPS on %d threads (%s)" % (cpu["value"], cpu["cores"], cpu["sample"]))
+
+    line = {
+        "metric": "GCUPS fwd+rev at 1 GPU, batched vs plain, bit-exact; p99 latency under load",
+        "value": round(batched["gcups"], 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batched["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic",
+        "config": {"workload": "batched uneven-length DNA mix: %d seeded pairs, query lognormal(median 150) "
+                               "[32,1024], ref lognormal(median 600) [64,2048], +3/-3/-6/-1" % w.n,
+                   "pairs": w.n, "cells": cells, "form": "batched"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "device_gcups": round(batched["device_gcups"], 2),
+        "plain": None if plain is None else {"gcups": round(plain["gcups"], 2),
+                                              "device_gcups": round(plain["device_gcups"], 2),
+                                              "ms_per_step": round(plain["ms_per_step"], 3)},
+        "batched_over_plain": None if plain is None else round(batched["gcups"] / plain["gcups"], 3),
+        "bit_exact_plain_vs_batched": plain is not None,
+    }
+    print(json.dumps(line), flush=True)
+    batch.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

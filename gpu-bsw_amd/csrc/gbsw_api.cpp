@@ -1,0 +1,516 @@
+// gbsw_api.cpp -- C ABI of the engine (include/gbsw.h): contexts, resident
+// batches, length binning and kernel launches.
+//
+// Replaces the host pipeline of driver.cpp:15-243 / 245-489 (packing, chunking,
+// two-stream copies, host newMin between the passes, pinned result arrays) with:
+// a batch uploaded once into HBM, a forward->reverse hand-off that never leaves
+// the device, and status codes instead of exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gbsw.h"
+#include "gbsw_kernels.hpp"
+
+using namespace gbsw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(GBSW_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+// Residue order of the reference's protein table (driver.cpp:252-258 maps each
+// of these letters to its index; every other byte below 91 maps to 0).
+const char kAaOrder[] = "ARNDCQEGHILKMFPSTWYVBZX*";
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    ~DeviceBuffer() { release(); }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    hipError_t reserve(size_t b)
+    {
+        if (b <= bytes) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&ptr, b);
+        if (e == hipSuccess) bytes = b;
+        else ptr = nullptr;
+        return e;
+    }
+    template <class T> T* as() const { return (T*)ptr; }
+};
+
+// One length bucket of the batched form: a kernel geometry and the slice of the
+// sorted order array it processes.
+struct Bucket {
+    int K = 0;
+    int64_t begin = 0, count = 0; // slots in the order array
+    int64_t cells = 0;
+};
+
+} // namespace
+
+struct gbsw_context {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::vector<hipEvent_t> kev; // per-launch event pairs
+    DeviceBuffer bnd, queue, status, sub, lut;
+};
+
+struct gbsw_batch {
+    gbsw_context* ctx = nullptr;
+    int32_t alphabet = 0;
+    int64_t n = 0;
+    DeviceBuffer ref, ref_off, qry, qry_off, out, order;
+    std::vector<int32_t> len_ref, len_qry;
+    int32_t alpha = 0;           // residue codes in use
+    uint8_t lut[256];            // byte -> code, 0xFF outside the alphabet
+    int32_t max_rows = 0;        // longest sequence
+    int32_t max_cols = 0;        // longest shorter-sequence of a pair
+    int64_t cells = 0;
+    // batched form: buckets over the sorted order array (built lazily)
+    std::vector<Bucket> buckets;
+    int buckets_for_maxk = -1;
+    gbsw_stats stats{};
+};
+
+namespace {
+
+int lds_for(int K, int alpha) { return lds_layout(kVirtPerPair * K, alpha).total; }
+
+// Largest K whose LDS footprint keeps at least two workgroups per CU.
+int max_k_for(int alpha)
+{
+    int best = 2;
+    for (int K : {2, 4, 8, 16, 32})
+        if (lds_for(K, alpha) <= 65536) best = K;
+    return best;
+}
+
+int k_for_cols(int cols, int kmax)
+{
+    for (int K : {2, 4, 8, 16, 32})
+        if (K <= kmax && kVirtPerPair * K >= cols) return K;
+    return kmax;
+}
+
+int grid_for(gbsw_context* ctx, int kind, int K, int lds, int64_t nquads)
+{
+    int occ = kernel_occupancy(kind, K, lds);
+    if (occ < 1) occ = 1;
+    const int64_t full = (int64_t)occ * ctx->num_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(full, nquads));
+}
+
+int build_buckets(gbsw_batch* b)
+{
+    const int kmax = max_k_for(b->alpha);
+    if (b->buckets_for_maxk == kmax) return GBSW_OK;
+    // bucket by the K that covers the shorter sequence; inside a bucket the
+    // longest pairs go first (longest-processing-time order for the queue)
+    std::vector<int32_t> idx((size_t)b->n);
+    std::vector<int8_t> kb((size_t)b->n);
+    for (int64_t p = 0; p < b->n; ++p) {
+        idx[(size_t)p] = (int32_t)p;
+        const int m = std::min(b->len_ref[(size_t)p], b->len_qry[(size_t)p]);
+        kb[(size_t)p] = (int8_t)k_for_cols(m, kmax);
+    }
+    std::sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+        if (kb[(size_t)x] != kb[(size_t)y]) return kb[(size_t)x] < kb[(size_t)y];
+        const int64_t cx = (int64_t)b->len_ref[(size_t)x] * b->len_qry[(size_t)x];
+        const int64_t cy = (int64_t)b->len_ref[(size_t)y] * b->len_qry[(size_t)y];
+        const int32_t mx = std::max(b->len_ref[(size_t)x], b->len_qry[(size_t)x]);
+        const int32_t my = std::max(b->len_ref[(size_t)y], b->len_qry[(size_t)y]);
+        if (mx != my) return mx > my;
+        if (cx != cy) return cx > cy;
+        return x < y;
+    });
+    b->buckets.clear();
+    for (int64_t s = 0; s < b->n;) {
+        Bucket bk;
+        bk.K = kb[(size_t)idx[(size_t)s]];
+        bk.begin = s;
+        int64_t e = s;
+        while (e < b->n && kb[(size_t)idx[(size_t)e]] == bk.K) {
+            bk.cells += (int64_t)b->len_ref[(size_t)idx[(size_t)e]] * b->len_qry[(size_t)idx[(size_t)e]];
+            ++e;
+        }
+        bk.count = e - s;
+        b->buckets.push_back(bk);
+        s = e;
+    }
+    HIP_TRY(b->order.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(1, b->n)));
+    if (b->n > 0)
+        HIP_TRY(hipMemcpy(b->order.ptr, idx.data(), sizeof(int32_t) * (size_t)b->n, hipMemcpyHostToDevice));
+    b->buckets_for_maxk = kmax;
+    return GBSW_OK;
+}
+
+int validate_scoring(const gbsw_batch* b, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub)
+{
+    if (sc->alphabet != b->alphabet) return fail(GBSW_E_ARGUMENT, "scoring alphabet differs from the batch alphabet");
+    if (sc->gap_open > 0 || sc->gap_extend > 0)
+        return fail(GBSW_E_SCORING, "gap_open and gap_extend must be <= 0");
+    const int A = b->alpha;
+    sub.assign((size_t)std::max(1, A * A), 0);
+    int mx = 0;
+    if (b->alphabet == GBSW_DNA) {
+        for (int r = 0; r < A; ++r)
+            for (int c = 0; c < A; ++c) sub[(size_t)(r * A + c)] = r == c ? sc->match : sc->mismatch;
+        mx = std::max<int>(sc->match, sc->mismatch);
+    } else {
+        for (int k = 0; k < 576; ++k) {
+            sub[(size_t)k] = sc->matrix[k];
+            mx = std::max<int>(mx, sc->matrix[k]);
+        }
+    }
+    for (int16_t v : sub)
+        if (v < -16000 || v > 16000) return fail(GBSW_E_SCORING, "substitution scores must lie in [-16000, 16000]");
+    *max_sub = std::max(0, mx);
+    return GBSW_OK;
+}
+
+KernelArgs base_args(gbsw_batch* b, const gbsw_scoring* sc, int max_sub)
+{
+    gbsw_context* ctx = b->ctx;
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.ref = b->ref.as<uint8_t>();
+    a.ref_off = b->ref_off.as<uint64_t>();
+    a.qry = b->qry.as<uint8_t>();
+    a.qry_off = b->qry_off.as<uint64_t>();
+    a.n = b->n;
+    a.out = b->out.as<int32_t>();
+    a.status = ctx->status.as<int32_t>();
+    a.lut = ctx->lut.as<uint8_t>();
+    a.sub = ctx->sub.as<int16_t>();
+    a.alpha = b->alpha;
+    a.gap_open = sc->gap_open;
+    a.gap_extend = sc->gap_extend;
+    a.sat_limit = 32767 - max_sub;
+    a.bnd_rows = std::max(1, b->max_rows);
+    return a;
+}
+
+int launch_timed(gbsw_context* ctx, int kind, int K, int grid, int lds, KernelArgs& a, size_t ev_slot)
+{
+    a.bnd = ctx->bnd.as<uint32_t>();
+    const size_t need = (size_t)grid * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t);
+    HIP_TRY(ctx->bnd.reserve(need));
+    a.bnd = ctx->bnd.as<uint32_t>();
+    while (ctx->kev.size() < 2 * (ev_slot + 1)) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        ctx->kev.push_back(e);
+    }
+    HIP_TRY(hipEventRecord(ctx->kev[2 * ev_slot], ctx->stream));
+    HIP_TRY(kernel_launch(kind, K, grid, lds, ctx->stream, a));
+    HIP_TRY(hipEventRecord(ctx->kev[2 * ev_slot + 1], ctx->stream));
+    return GBSW_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* gbsw_last_error(void) { return g_err.c_str(); }
+
+const char* gbsw_version(void) { return "gbsw-mi355x 0.1 (gfx950, packed int16 wavefront)"; }
+
+int gbsw_device_memory(int32_t device, uint64_t* bytes)
+{
+    if (!bytes) return fail(GBSW_E_ARGUMENT, "bytes is null");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    *bytes = (uint64_t)prop.totalGlobalMem;
+    return GBSW_OK;
+}
+
+int gbsw_context_create(int32_t device, gbsw_context** out)
+{
+    if (!out) return fail(GBSW_E_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(GBSW_E_ARGUMENT, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    gbsw_context* ctx = new gbsw_context();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stop);
+    if (e == hipSuccess) e = ctx->status.reserve(sizeof(int32_t) * kStatusWords);
+    if (e == hipSuccess) e = ctx->queue.reserve(sizeof(uint32_t) * 64);
+    if (e == hipSuccess) e = ctx->sub.reserve(sizeof(int16_t) * 1024);
+    if (e == hipSuccess) e = ctx->lut.reserve(256);
+    if (e != hipSuccess) {
+        gbsw_context_destroy(ctx);
+        return fail(GBSW_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    ctx->num_cu = prop.multiProcessorCount;
+    *out = ctx;
+    return GBSW_OK;
+}
+
+int gbsw_context_destroy(gbsw_context* ctx)
+{
+    if (!ctx) return GBSW_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    ctx->bnd.release(); ctx->queue.release(); ctx->status.release(); ctx->sub.release(); ctx->lut.release();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GBSW_OK;
+}
+
+int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char* ref,
+                      const uint64_t* ref_off, const char* query, const uint64_t* query_off,
+                      gbsw_batch** out)
+{
+    if (!ctx || !out || n < 0 || (n > 0 && (!ref_off || !query_off)))
+        return fail(GBSW_E_ARGUMENT, "null argument or negative pair count");
+    if (alphabet != GBSW_DNA && alphabet != GBSW_PROTEIN) return fail(GBSW_E_ARGUMENT, "unknown alphabet");
+    if (n > (int64_t)0x7FFFFFFF) return fail(GBSW_E_RANGE, "more than 2^31-1 pairs in one batch");
+    *out = nullptr;
+    const uint64_t ref_bytes = n ? ref_off[n] - ref_off[0] : 0;
+    const uint64_t qry_bytes = n ? query_off[n] - query_off[0] : 0;
+    if ((ref_bytes && !ref) || (qry_bytes && !query)) return fail(GBSW_E_ARGUMENT, "sequence buffer is null");
+    HIP_TRY(hipSetDevice(ctx->device));
+
+    gbsw_batch* b = new gbsw_batch();
+    b->ctx = ctx;
+    b->alphabet = alphabet;
+    b->n = n;
+    b->len_ref.resize((size_t)n);
+    b->len_qry.resize((size_t)n);
+    std::vector<uint64_t> ro((size_t)n + 1), qo((size_t)n + 1);
+    for (int64_t p = 0; p <= n; ++p) {
+        ro[(size_t)p] = n ? ref_off[p] - ref_off[0] : 0;
+        qo[(size_t)p] = n ? query_off[p] - query_off[0] : 0;
+    }
+    for (int64_t p = 0; p < n; ++p) {
+        if (ro[(size_t)p + 1] < ro[(size_t)p] || qo[(size_t)p + 1] < qo[(size_t)p]) {
+            delete b;
+            return fail(GBSW_E_ARGUMENT, "offsets must be non-decreasing");
+        }
+        const uint64_t lr = ro[(size_t)p + 1] - ro[(size_t)p], lq = qo[(size_t)p + 1] - qo[(size_t)p];
+        if (lr > 65000 || lq > 65000) {
+            delete b;
+            return fail(GBSW_E_RANGE, "sequences longer than 65000 residues are not supported");
+        }
+        b->len_ref[(size_t)p] = (int32_t)lr;
+        b->len_qry[(size_t)p] = (int32_t)lq;
+        b->max_rows = std::max<int32_t>(b->max_rows, (int32_t)std::max(lr, lq));
+        b->max_cols = std::max<int32_t>(b->max_cols, (int32_t)std::min(lr, lq));
+        b->cells += (int64_t)(lr * lq);
+    }
+    // residue alphabet: protein uses the reference encoding, DNA the set of bytes present
+    std::memset(b->lut, 0xFF, sizeof(b->lut));
+    const uint8_t* rp = (const uint8_t*)ref + (n ? ref_off[0] : 0);
+    const uint8_t* qp = (const uint8_t*)query + (n ? query_off[0] : 0);
+    if (alphabet == GBSW_PROTEIN) {
+        for (int c = 0; c < 91; ++c) {
+            const char* f = std::strchr(kAaOrder, c);
+            b->lut[c] = (uint8_t)((f && c) ? (f - kAaOrder) : 0);
+        }
+        b->alpha = 24;
+        bool bad = false;
+        for (uint64_t i = 0; i < ref_bytes && !bad; ++i) bad = rp[i] >= 91;
+        for (uint64_t i = 0; i < qry_bytes && !bad; ++i) bad = qp[i] >= 91;
+        if (bad) {
+            delete b;
+            return fail(GBSW_E_RESIDUE, "protein residue byte >= 91 (the reference encoding table has 91 entries)");
+        }
+    } else {
+        bool seen[256] = {false};
+        for (uint64_t i = 0; i < ref_bytes; ++i) seen[rp[i]] = true;
+        for (uint64_t i = 0; i < qry_bytes; ++i) seen[qp[i]] = true;
+        int A = 0;
+        for (int c = 0; c < 256; ++c)
+            if (seen[c]) b->lut[c] = (uint8_t)A++;
+        if (A > kMaxAlpha) {
+            delete b;
+            return fail(GBSW_E_RESIDUE, "more than 31 distinct DNA bytes in one batch");
+        }
+        b->alpha = std::max(A, 1);
+    }
+    hipError_t e = hipSuccess;
+    e = b->ref.reserve(std::max<uint64_t>(ref_bytes, 16));
+    if (e == hipSuccess) e = b->qry.reserve(std::max<uint64_t>(qry_bytes, 16));
+    if (e == hipSuccess) e = b->ref_off.reserve(sizeof(uint64_t) * (size_t)(n + 1));
+    if (e == hipSuccess) e = b->qry_off.reserve(sizeof(uint64_t) * (size_t)(n + 1));
+    if (e == hipSuccess) e = b->out.reserve(sizeof(int32_t) * 5 * (size_t)std::max<int64_t>(n, 1));
+    if (e == hipSuccess && ref_bytes) e = hipMemcpy(b->ref.ptr, rp, ref_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && qry_bytes) e = hipMemcpy(b->qry.ptr, qp, qry_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->ref_off.ptr, ro.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->qry_off.ptr, qo.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->out.ptr, 0, sizeof(int32_t) * 5 * (size_t)std::max<int64_t>(n, 1));
+    if (e != hipSuccess) {
+        delete b;
+        return fail(e == hipErrorOutOfMemory ? GBSW_E_MEMORY : GBSW_E_DEVICE,
+                    std::string("batch upload: ") + hipGetErrorString(e));
+    }
+    b->stats.cells = b->cells;
+    *out = b;
+    return GBSW_OK;
+}
+
+int gbsw_batch_destroy(gbsw_batch* b)
+{
+    if (!b) return GBSW_OK;
+    (void)hipSetDevice(b->ctx->device);
+    delete b;
+    return GBSW_OK;
+}
+
+int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
+{
+    if (!b || !sc) return fail(GBSW_E_ARGUMENT, "null argument");
+    if (mode != GBSW_PLAIN && mode != GBSW_BATCHED) return fail(GBSW_E_ARGUMENT, "unknown mode");
+    gbsw_context* ctx = b->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<int16_t> sub;
+    int max_sub = 0;
+    int rc = validate_scoring(b, sc, sub, &max_sub);
+    if (rc) return rc;
+    // int16 cells are exact while no score can reach the saturation limit
+    if ((int64_t)b->max_cols * max_sub >= 32767 - max_sub)
+        return fail(GBSW_E_RANGE, "possible score exceeds the int16 range of this engine");
+    b->stats = gbsw_stats{};
+    b->stats.cells = b->cells;
+    if (b->n == 0) return GBSW_OK;
+
+    HIP_TRY(hipMemcpyAsync(ctx->sub.ptr, sub.data(), sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->lut.ptr, b->lut, 256, hipMemcpyHostToDevice, ctx->stream));
+    KernelArgs a = base_args(b, sc, max_sub);
+    const int kmax = max_k_for(b->alpha);
+    std::vector<std::pair<int, int>> timed; // (event slot, is main kernel)
+    HIP_TRY(zero_u32((uint32_t*)ctx->status.ptr, kStatusWords, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+    size_t slot = 0;
+    if (mode == GBSW_PLAIN) {
+        // one geometry for the whole batch, sized by its longest shorter sequence
+        const int K = k_for_cols(b->max_cols, kmax);
+        const int lds = lds_for(K, b->alpha);
+        const int64_t nquads = (b->n + kPairsPerWave - 1) / kPairsPerWave;
+        a.order = nullptr;
+        a.order_n = b->n;
+        const int gf = grid_for(ctx, kPlainForward, K, lds, nquads);
+        rc = launch_timed(ctx, kPlainForward, K, gf, lds, a, slot++);
+        if (rc) return rc;
+        const int gr = grid_for(ctx, kPlainReverse, K, lds, nquads);
+        rc = launch_timed(ctx, kPlainReverse, K, gr, lds, a, slot++);
+        if (rc) return rc;
+        b->stats.buckets = 1;
+    } else {
+        rc = build_buckets(b);
+        if (rc) return rc;
+        if (ctx->queue.bytes < sizeof(uint32_t) * b->buckets.size())
+            HIP_TRY(ctx->queue.reserve(sizeof(uint32_t) * b->buckets.size()));
+        HIP_TRY(zero_u32((uint32_t*)ctx->queue.ptr, (int)b->buckets.size(), ctx->stream));
+        for (size_t i = 0; i < b->buckets.size(); ++i) {
+            const Bucket& bk = b->buckets[i];
+            const int lds = lds_for(bk.K, b->alpha);
+            const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
+            a.order = b->order.as<int32_t>() + bk.begin;
+            a.order_n = bk.count;
+            a.queue = (uint32_t*)ctx->queue.ptr + i;
+            const int grid = grid_for(ctx, kBatched, bk.K, lds, nquads);
+            rc = launch_timed(ctx, kBatched, bk.K, grid, lds, a, slot++);
+            if (rc) return rc;
+        }
+        b->stats.buckets = (int32_t)b->buckets.size();
+    }
+    HIP_TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int32_t status[kStatusWords];
+    HIP_TRY(hipMemcpy(status, ctx->status.ptr, sizeof(status), hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    b->stats.device_ms = ms;
+    b->stats.launches = (int32_t)slot;
+    double kms = 0.0;
+    for (size_t i = 0; i < slot; ++i) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
+        kms += t;
+    }
+    b->stats.main_kernel_ms = kms;
+    if (status[kStatusSaturated]) return fail(GBSW_E_RANGE, "a score reached the int16 saturation limit");
+    if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
+    return GBSW_OK;
+}
+
+int gbsw_batch_results(gbsw_batch* b, int32_t* score, int32_t* ref_begin, int32_t* ref_end,
+                       int32_t* query_begin, int32_t* query_end)
+{
+    if (!b) return fail(GBSW_E_ARGUMENT, "batch is null");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    int32_t* dst[5] = {score, ref_begin, ref_end, query_begin, query_end};
+    for (int f = 0; f < 5; ++f)
+        if (dst[f] && b->n)
+            HIP_TRY(hipMemcpy(dst[f], b->out.as<int32_t>() + (size_t)f * (size_t)b->n, sizeof(int32_t) * (size_t)b->n,
+                              hipMemcpyDeviceToHost));
+    return GBSW_OK;
+}
+
+int gbsw_batch_device_results(gbsw_batch* b, const int32_t** d_out)
+{
+    if (!b || !d_out) return fail(GBSW_E_ARGUMENT, "null argument");
+    *d_out = b->out.as<int32_t>();
+    return GBSW_OK;
+}
+
+int gbsw_batch_stats(const gbsw_batch* b, gbsw_stats* out)
+{
+    if (!b || !out) return fail(GBSW_E_ARGUMENT, "null argument");
+    *out = b->stats;
+    return GBSW_OK;
+}
+
+int gbsw_align(gbsw_context* ctx, const gbsw_scoring* sc, int32_t mode, int64_t n, const char* ref,
+               const uint64_t* ref_off, const char* query, const uint64_t* query_off, int32_t* score,
+               int32_t* ref_begin, int32_t* ref_end, int32_t* query_begin, int32_t* query_end)
+{
+    if (!sc) return fail(GBSW_E_ARGUMENT, "scoring is null");
+    gbsw_batch* b = nullptr;
+    int rc = gbsw_batch_create(ctx, sc->alphabet, n, ref, ref_off, query, query_off, &b);
+    if (rc) return rc;
+    rc = gbsw_batch_align(b, sc, mode);
+    if (!rc) rc = gbsw_batch_results(b, score, ref_begin, ref_end, query_begin, query_end);
+    const std::string err = g_err;
+    gbsw_batch_destroy(b);
+    g_err = err;
+    return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,179 @@
+// gbsw_driver.cpp -- gpu_bsw_driver drop-in (include/gpu_bsw_driver.hpp) over the C ABI.
+//
+// Mirrors the reference driver's contract (driver.cpp:15-243, 245-489;
+// utils.cpp:13-28): string vectors in, five pinned `short` arrays out, the
+// "Total ..." summary lines on stdout.  Packing, device transfer and both passes
+// go through gbsw_align (one resident batch per call).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gbsw.h"
+#include "../../include/gpu_bsw_driver.hpp"
+
+namespace {
+
+thread_local int g_status = 0;
+std::mutex g_ctx_mu;
+gbsw_context* g_ctx = nullptr;
+
+gbsw_context* shared_context(int* rc)
+{
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (g_ctx) { *rc = GBSW_OK; return g_ctx; }
+    int dev = 0;
+    if (const char* e = std::getenv("GBSW_DEVICE")) dev = std::atoi(e);
+    else (void)hipGetDevice(&dev);
+    *rc = gbsw_context_create(dev, &g_ctx);
+    return *rc == GBSW_OK ? g_ctx : nullptr;
+}
+
+int32_t driver_mode()
+{
+    const char* e = std::getenv("GBSW_MODE");
+    return (e && std::strcmp(e, "plain") == 0) ? GBSW_PLAIN : GBSW_BATCHED;
+}
+
+// initialize_alignments (utils.cpp:13-19): pinned host arrays owned by the caller.
+bool allocate_results(gpu_bsw_driver::alignment_results* r, size_t n)
+{
+    short** f[5] = {&r->ref_begin, &r->ref_end, &r->query_begin, &r->query_end, &r->top_scores};
+    const size_t bytes = sizeof(short) * (n ? n : 1);
+    for (short** p : f) {
+        *p = nullptr;
+        if (hipHostMalloc((void**)p, bytes, hipHostMallocDefault) != hipSuccess) return false;
+        std::memset(*p, 0, bytes);
+    }
+    return true;
+}
+
+void pack(const std::vector<std::string>& v, std::vector<char>& bytes, std::vector<uint64_t>& off, size_t n)
+{
+    off.resize(n + 1);
+    off[0] = 0;
+    for (size_t k = 0; k < n; ++k) off[k + 1] = off[k] + v[k].size();
+    bytes.resize(off[n] ? off[n] : 1);
+    for (size_t k = 0; k < n; ++k)
+        if (!v[k].empty()) std::memcpy(bytes.data() + off[k], v[k].data(), v[k].size());
+}
+
+[[noreturn]] void die(int rc)
+{
+    std::fprintf(stderr, "GPUassert: %s (gbsw status %d)\n", gbsw_last_error(), rc);
+    std::exit(1);
+}
+
+void run(std::vector<std::string>& reads, std::vector<std::string>& contigs,
+         gpu_bsw_driver::alignment_results* alignments, const gbsw_scoring& sc)
+{
+    const size_t n = contigs.size(); // driver.cpp:21 pairs reads[k] with contigs[k]
+    g_status = GBSW_OK;
+    if (!allocate_results(alignments, n)) {
+        g_status = GBSW_E_MEMORY;
+        if (!std::getenv("GBSW_DRIVER_NOEXIT")) die(g_status);
+        return;
+    }
+    if (reads.size() < n) {
+        g_status = GBSW_E_ARGUMENT;
+        if (!std::getenv("GBSW_DRIVER_NOEXIT")) {
+            std::fprintf(stderr, "GPUassert: fewer reads than contigs\n");
+            std::exit(1);
+        }
+        return;
+    }
+    size_t max_ref = 0, max_qry = 0;
+    for (size_t k = 0; k < n; ++k) {
+        max_ref = std::max(max_ref, contigs[k].size());
+        max_qry = std::max(max_qry, reads[k].size());
+    }
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    int rc = GBSW_OK;
+    gbsw_context* ctx = shared_context(&rc);
+    std::vector<char> rb, qb;
+    std::vector<uint64_t> ro, qo;
+    std::vector<int32_t> out(5 * (n ? n : 1));
+    if (rc == GBSW_OK) {
+        pack(contigs, rb, ro, n);
+        pack(reads, qb, qo, n);
+        rc = gbsw_align(ctx, &sc, driver_mode(), (int64_t)n, rb.data(), ro.data(), qb.data(), qo.data(),
+                        out.data(), out.data() + n, out.data() + 2 * n, out.data() + 3 * n, out.data() + 4 * n);
+    }
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    g_status = rc;
+    if (rc != GBSW_OK) {
+        if (!std::getenv("GBSW_DRIVER_NOEXIT")) die(rc);
+        return;
+    }
+    for (size_t k = 0; k < n; ++k) {
+        alignments->top_scores[k] = (short)out[k];
+        alignments->ref_begin[k] = (short)out[n + k];
+        alignments->ref_end[k] = (short)out[2 * n + k];
+        alignments->query_begin[k] = (short)out[3 * n + k];
+        alignments->query_end[k] = (short)out[4 * n + k];
+    }
+    const std::chrono::duration<double> dt = t1 - t0;
+    std::cout << "Total Alignments:" << n << "\n"
+              << "Max Reference Size:" << max_ref << "\n"
+              << "Max Query Size:" << max_qry << "\n"
+              << "Total Execution Time (seconds):" << dt.count() << std::endl;
+}
+
+} // namespace
+
+namespace gpu_bsw_driver {
+
+size_t get_tot_gpu_mem(int id)
+{
+    uint64_t b = 0;
+    if (gbsw_device_memory(id, &b) != GBSW_OK) return 0;
+    return (size_t)b;
+}
+
+void kernel_driver_dna(std::vector<std::string> reads, std::vector<std::string> contigs,
+                       alignment_results* alignments, short scores[4], float factor)
+{
+    (void)factor;
+    gbsw_scoring sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.alphabet = GBSW_DNA;
+    sc.match = scores[0];      // driver.cpp:18
+    sc.mismatch = scores[1];
+    sc.gap_open = scores[2];
+    sc.gap_extend = scores[3];
+    run(reads, contigs, alignments, sc);
+}
+
+void kernel_driver_aa(std::vector<std::string> reads, std::vector<std::string> contigs,
+                      alignment_results* alignments, short scoring_matrix[], short openGap, short extendGap,
+                      float factor)
+{
+    (void)factor;
+    gbsw_scoring sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.alphabet = GBSW_PROTEIN;
+    sc.gap_open = openGap;
+    sc.gap_extend = extendGap;
+    std::memcpy(sc.matrix, scoring_matrix, sizeof(sc.matrix)); // SCORE_MAT_SIZE = 576 (kernel.hpp:9)
+    run(reads, contigs, alignments, sc);
+}
+
+int last_status() { return g_status; }
+
+} // namespace gpu_bsw_driver
+
+// utils.cpp:21-28
+void free_alignments(gpu_bsw_driver::alignment_results* alignments)
+{
+    if (!alignments) return;
+    short* f[5] = {alignments->ref_begin, alignments->ref_end, alignments->query_begin, alignments->query_end,
+                   alignments->top_scores};
+    for (short* p : f)
+        if (p) (void)hipHostFree(p);
+}

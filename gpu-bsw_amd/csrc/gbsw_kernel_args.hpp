@@ -1,0 +1,67 @@
+// gbsw_kernel_args.hpp -- argument block and LDS layout shared by host and kernels.
+#pragma once
+#include <stdint.h>
+
+namespace gbsw {
+
+constexpr int kLanesPerPair = 16;              // one DPP row
+constexpr int kVirtPerPair = 2 * kLanesPerPair; // two int16 halves per lane
+constexpr int kPairsPerWave = 4;
+constexpr int kWave = 64;
+constexpr int kRing = 128;   // row-stream ring entries per pair
+constexpr int kChunk = 32;   // rows refilled at a time
+constexpr int kMaxAlpha = 31; // codes 0..A-1, row A is the padding row
+constexpr int16_t kNeg = -16384;
+
+// Output layout: five int32 arrays of n entries, back to back.
+enum { kOutScore = 0, kOutRefBegin = 1, kOutRefEnd = 2, kOutQueryBegin = 3, kOutQueryEnd = 4 };
+
+// status[] words written by kernels
+enum { kStatusBadResidue = 0, kStatusSaturated = 1, kStatusRevMiss = 2, kStatusWords = 4 };
+
+struct KernelArgs {
+    const uint8_t* ref;
+    const uint64_t* ref_off;   // n+1 exclusive prefix offsets
+    const uint8_t* qry;
+    const uint64_t* qry_off;
+    int64_t n;                 // pairs in the batch
+    const int32_t* order;      // pair index of each work slot (nullptr = identity)
+    int64_t order_n;           // slots in this launch
+    uint32_t* queue;           // persistent work counter (batched)
+    int32_t* out;              // 5*n int32
+    uint32_t* bnd;             // strip boundary rows, kPairsPerWave*bnd_rows per workgroup
+    int64_t bnd_rows;
+    int32_t* status;
+    const uint8_t* lut;        // byte -> residue code, 0xFF for bytes outside the alphabet
+    const int16_t* sub;        // A*A scores, index row_code*A + col_code
+    int32_t alpha;             // A
+    int32_t gap_open;          // <= 0
+    int32_t gap_extend;        // <= 0
+    int32_t sat_limit;         // scores >= this may have saturated int16
+};
+
+// Dynamic LDS layout of one 64-lane workgroup for a strip width W.
+struct LdsLayout {
+    int32_t lut_off, sub_off, xch_off, info_off, group_off, group_bytes;
+    int32_t prof_bytes, ring_code_off, ring_bnd_off, total;
+};
+
+__host__ __device__ inline int32_t align_up(int32_t x, int32_t a) { return (x + a - 1) / a * a; }
+
+__host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha)
+{
+    LdsLayout L;
+    L.lut_off = 0;                                       // 256 B
+    L.sub_off = 256;                                     // kMaxAlpha^2 int16
+    L.xch_off = align_up(L.sub_off + 2 * 32 * 32, 16);   // 64 lanes x 2 keys x 8 B
+    L.info_off = L.xch_off + 64 * 2 * 8;                 // 4 groups x 8 int32
+    L.group_off = align_up(L.info_off + 4 * 8 * 4, 128);
+    L.prof_bytes = align_up((alpha + 1) * W * 2, 128);
+    L.ring_code_off = L.prof_bytes;
+    L.ring_bnd_off = L.ring_code_off + kRing * 2;
+    L.group_bytes = align_up(L.ring_bnd_off + kRing * 4, 128);
+    L.total = L.group_off + kPairsPerWave * L.group_bytes;
+    return L;
+}
+
+} // namespace gbsw

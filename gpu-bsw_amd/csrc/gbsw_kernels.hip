@@ -1,0 +1,576 @@
+// gbsw_kernels.hip -- gfx950 kernels of the batched Smith-Waterman engine.
+//
+// Replaces the reference's sequence_{dna,aa}_{kernel,reverse} (kernel.cpp:198-1059)
+// with one wavefront engine for both alphabets.
+//
+// Geometry.  A wave64 aligns four pairs, one per 16-lane DPP row ("group").  Each
+// lane owns two column blocks of K columns of the shorter sequence (the
+// reference's column orientation, kernel.cpp:248-259): the low int16 half of every
+// state register holds block l ("virtual lane" v = l), the high half block l+16
+// (v = l+16).  Virtual lane v works on row s-v at step s, so the 32 virtual lanes
+// of a pair form an anti-diagonal wavefront over a strip of W = 32*K columns, and
+// the two halves of a register are always independent cells: every v_pk_* op
+// advances two DP cells.  Longer column sequences are processed strip by strip;
+// strip t+1 reads the last column of strip t (H and E per row) from global memory.
+//
+// Recurrence (same H as kernel.cpp:329-367; E and F are kept clamped at 0, which
+// cannot change H = max(D, E, F, 0) when the gap scores are <= 0):
+//   D  = H(i-1,j-1) + s(row, col)                      v_pk_add_i16 clamp
+//   H  = max(D, E', F')                                2 x v_pk_max_i16
+//   Ho = max(H + open, 0)                              v_pk_sub_u16 clamp
+//   E' = max(max(E'+ext, 0), Ho)   (next column)       v_pk_sub_u16 clamp + max
+//   F' = max(max(F'+ext, 0), Ho)   (next row)          v_pk_sub_u16 clamp + max
+// Substitution scores come from a per-strip profile in LDS: row a of the profile
+// holds s(code a, column j) for the strip's columns; row A is the padding row.
+//
+// Results follow the reference's tie rules exactly:
+//  forward: among cells with H = score, the smallest row, then the smallest tree
+//    priority of the column (kernel.cpp:47-86, 137-182, 368-370).  Each virtual lane
+//    keeps (best, first row reaching it, snapshot of that row); the winner is
+//    chosen after the pass.
+//  reverse: the same DP on the reversed prefixes; among cells with H = score, the
+//    smallest reversed column, then its first row (kernel.cpp:3-45, 577-581).
+#include "gbsw_device.hpp"
+#include "gbsw_kernel_args.hpp"
+#include "gbsw_kernels.hpp"
+
+namespace gbsw {
+
+// One pass (forward or reverse) of one pair, identical in the 16 lanes of a group.
+struct Task {
+    const uint8_t* rows;  // row sequence (the longer one of this pass)
+    const uint8_t* cols;  // column sequence
+    int32_t nr, nc;
+    int32_t rev;          // 1: both sequences are read backwards (reverse pass)
+};
+
+struct Smem {
+    uint8_t* base;
+    LdsLayout L;
+    __device__ uint8_t* lut() const { return base + L.lut_off; }
+    __device__ int16_t* sub() const { return (int16_t*)(base + L.sub_off); }
+    __device__ uint64_t* xch() const { return (uint64_t*)(base + L.xch_off); }
+    __device__ int32_t* info() const { return (int32_t*)(base + L.info_off); }
+    __device__ uint8_t* group(int g) const { return base + L.group_off + g * L.group_bytes; }
+};
+
+__device__ __forceinline__ uint32_t inv_prio(uint32_t p)
+{
+    // tie_prio is a per-1024-block bit-reversal of two 5-bit fields: its own inverse.
+    return tie_prio(p);
+}
+
+// Raw bytes of 2 ring rows prefetched 32 steps before they are stored in LDS.
+struct RowFetch {
+    int32_t b0, b1;     // residue byte, -1 outside [0, nr)
+    uint32_t d0, d1;    // strip boundary (H | E << 16)
+};
+
+__device__ __forceinline__ uint32_t load_coherent(const uint32_t* p)
+{
+    // The boundary rows were stored by this workgroup during the previous strip;
+    // an agent-scope load does not hit a stale L1 line.
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ RowFetch row_fetch(const Task& t, int chunk, int l, const uint32_t* bnd, bool use_bnd)
+{
+    RowFetch f;
+    const int x0 = chunk * kChunk + l, x1 = x0 + 16;
+    const bool v0 = x0 >= 0 && x0 < t.nr, v1 = x1 >= 0 && x1 < t.nr;
+    f.b0 = v0 ? (int32_t)t.rows[t.rev ? t.nr - 1 - x0 : x0] : -1;
+    f.b1 = v1 ? (int32_t)t.rows[t.rev ? t.nr - 1 - x1 : x1] : -1;
+    f.d0 = (v0 && use_bnd) ? load_coherent(bnd + x0) : 0u;
+    f.d1 = (v1 && use_bnd) ? load_coherent(bnd + x1) : 0u;
+    return f;
+}
+
+__device__ __forceinline__ void row_store(const RowFetch& f, int chunk, int l, uint16_t* ring_code,
+                                          uint32_t* ring_bnd, const uint8_t* lut, int alpha, int rowstride)
+{
+    const int x0 = chunk * kChunk + l, x1 = x0 + 16;
+    int c0 = f.b0 < 0 ? alpha : (int)lut[f.b0];
+    int c1 = f.b1 < 0 ? alpha : (int)lut[f.b1];
+    c0 = c0 > alpha ? alpha : c0; // bytes outside the alphabet read the padding row
+    c1 = c1 > alpha ? alpha : c1;
+    ring_code[x0 & (kRing - 1)] = (uint16_t)(c0 * rowstride);
+    ring_code[x1 & (kRing - 1)] = (uint16_t)(c1 * rowstride);
+    ring_bnd[x0 & (kRing - 1)] = f.d0;
+    ring_bnd[x1 & (kRing - 1)] = f.d1;
+}
+
+// Load the K profile scores of one virtual lane: K int16 = K/2 dwords.
+template <int K>
+__device__ __forceinline__ void prof_load(const uint8_t* p, uint32_t (&d)[K / 2])
+{
+    if constexpr (K % 8 == 0) {
+#pragma unroll
+        for (int q = 0; q < K / 8; ++q) {
+            const uint4 v = *(const uint4*)(p + 16 * q);
+            d[4 * q + 0] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
+        }
+    } else if constexpr (K == 4) {
+        const uint2 v = *(const uint2*)p;
+        d[0] = v.x; d[1] = v.y;
+    } else {
+        static_assert(K == 2, "K must be 2, 4 or a multiple of 8");
+        d[0] = *(const uint32_t*)p;
+    }
+}
+
+struct PassResult {
+    uint64_t key; // 0: no cell reached a positive score (forward) / the target (reverse)
+};
+
+// Runs one pass for the four groups of the wave.  All loops are wave-uniform;
+// a group whose pair is shorter keeps computing on padding rows and columns,
+// which can neither exceed nor (by row/column order) win over its real cells.
+//
+// Forward key: (score << 40) | ((2^20-1-row) << 20) | (2^20-1-prio(col)), max wins.
+// Reverse key: ((2^20-1-col) << 20) | (2^20-1-row), max wins (target = score).
+template <int K, bool REV>
+__device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& t, int32_t target,
+                               uint32_t* bnd_g)
+{
+    constexpr int W = kVirtPerPair * K;
+    const int lane = threadIdx.x;
+    const int g = lane >> 4, l = lane & 15;
+    const int alpha = a.alpha;
+    const int rowstride = W * 2;
+    uint8_t* gb = sm.group(g);
+    int16_t* prof = (int16_t*)gb;
+    uint16_t* ring_code = (uint16_t*)(gb + sm.L.ring_code_off);
+    uint32_t* ring_bnd = (uint32_t*)(gb + sm.L.ring_bnd_off);
+    const uint8_t* lut = sm.lut();
+    const int16_t* sub = sm.sub();
+
+    // wave-uniform extents
+    const int strips_g = (t.nr > 0 && t.nc > 0) ? (t.nc + W - 1) / W : 0;
+    const int steps_g = t.nr + kVirtPerPair - 1;
+    int32_t* info = sm.info();
+    __syncthreads();
+    if (l == 0) { info[g * 8 + 0] = strips_g; info[g * 8 + 1] = steps_g; }
+    __syncthreads();
+    int strips_max = 0, steps_max = 0;
+#pragma unroll
+    for (int q = 0; q < kPairsPerWave; ++q) {
+        strips_max = max(strips_max, info[q * 8 + 0]);
+        steps_max = max(steps_max, info[q * 8 + 1]);
+    }
+
+    const uint32_t OPEN2 = pack16(-a.gap_open, -a.gap_open);
+    const uint32_t EXT2 = pack16(-a.gap_extend, -a.gap_extend);
+    const int v_lo = l, v_hi = l + 16;
+    uint64_t cand_lo = 0, cand_hi = 0;
+
+    for (int strip = 0; strip < strips_max; ++strip) {
+        const int strip_base = strip * W;
+        const bool live = strip < strips_g;
+        const bool use_bnd = strip > 0;
+        const bool write_bnd = live && strip + 1 < strips_g;
+
+        // ---- profile of this strip: prof[a*W + j] = s(code a, column strip_base+j)
+        __syncthreads();
+        for (int j = l; j < W; j += 16) {
+            const int c = strip_base + j;
+            int code = -1;
+            if (live && c < t.nc) {
+                code = lut[t.cols[t.rev ? t.nc - 1 - c : c]];
+                if (code >= alpha) code = -1;
+            }
+            for (int r = 0; r <= alpha; ++r)
+                prof[r * W + j] = (code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg;
+        }
+        // ---- row ring: rows [-32, 0) padding, [0, 32) now, [32, 64) in flight
+        {
+            RowFetch pad; pad.b0 = pad.b1 = -1; pad.d0 = pad.d1 = 0;
+            row_store(pad, -1, l, ring_code, ring_bnd, lut, alpha, rowstride);
+            RowFetch f0 = row_fetch(t, 0, l, bnd_g, use_bnd);
+            row_store(f0, 0, l, ring_code, ring_bnd, lut, alpha, rowstride);
+        }
+        RowFetch pf = row_fetch(t, 1, l, bnd_g, use_bnd);
+        __syncthreads();
+
+        const uint8_t* prof_lo = (const uint8_t*)prof + v_lo * K * 2;
+        const uint8_t* prof_hi = (const uint8_t*)prof + v_hi * K * 2;
+
+        uint32_t Hcol[K], Fcol[K];
+        uint32_t snap[REV ? 1 : K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) { Hcol[k] = 0; Fcol[k] = 0; }
+#pragma unroll
+        for (int k = 0; k < (REV ? 1 : K); ++k) snap[k] = 0;
+        uint32_t Hout = 0, Eout = 0, Hdiag0 = 0;
+        uint32_t best = 0, brow = 0;
+        uint32_t rowpk = pack16(-v_lo, -v_hi); // row of each half at step 0
+        int32_t rt_lo = 0x7FFFFFFF, rr_lo = 0, rt_hi = 0x7FFFFFFF, rr_hi = 0; // reverse hits
+
+        for (int s0 = 0; s0 < steps_max; s0 += kChunk) {
+            // refill: rows of chunk (s0/32 + 1) enter the ring, chunk +2 is requested
+            {
+                const int ck = s0 / kChunk + 1;
+                row_store(pf, ck, l, ring_code, ring_bnd, lut, alpha, rowstride);
+                pf = row_fetch(t, ck + 1, l, bnd_g, use_bnd);
+            }
+            __syncthreads();
+            const int s_end = min(s0 + kChunk, steps_max);
+            for (int s = s0; s < s_end; ++s) {
+                // left inputs: neighbour's outputs of the previous step; virtual lane 0
+                // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
+                uint32_t xH = row_ror1(Hout), xE = row_ror1(Eout);
+                const uint32_t bnd = ring_bnd[s & (kRing - 1)];
+                if (l == 0) {
+                    xH = __builtin_amdgcn_perm(xH, bnd, 0x05040100u);
+                    xE = __builtin_amdgcn_perm(xE, bnd, 0x05040302u);
+                }
+                const uint32_t off_lo = ring_code[(s - v_lo) & (kRing - 1)];
+                const uint32_t off_hi = ring_code[(s - v_hi) & (kRing - 1)];
+                uint32_t plo[K / 2], phi[K / 2];
+                prof_load<K>(prof_lo + off_lo, plo);
+                prof_load<K>(prof_hi + off_hi, phi);
+
+                uint32_t E = xE, Hd = Hdiag0;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t S = (k & 1) ? __builtin_amdgcn_perm(phi[k / 2], plo[k / 2], 0x07060302u)
+                                               : __builtin_amdgcn_perm(phi[k / 2], plo[k / 2], 0x05040100u);
+                    const uint32_t D = pk_add_sat(Hd, S);
+                    Hd = Hcol[k];
+                    const uint32_t H = pk_max(pk_max(D, E), Fcol[k]);
+                    const uint32_t Ho = pk_sub_floor0(H, OPEN2);
+                    E = pk_max(pk_sub_floor0(E, EXT2), Ho);
+                    Fcol[k] = pk_max(pk_sub_floor0(Fcol[k], EXT2), Ho);
+                    Hcol[k] = H;
+                }
+                Hout = Hcol[K - 1];
+                Eout = E;
+                Hdiag0 = xH;
+
+                // the last virtual lane hands its column to the next strip
+                if (write_bnd && l == 15) {
+                    const int row = s - v_hi;
+                    if (row >= 0 && row < t.nr)
+                        bnd_g[row] = __builtin_amdgcn_perm(Eout, Hout, 0x07060302u);
+                }
+
+                uint32_t rm = Hcol[0];
+#pragma unroll
+                for (int k = 1; k < K; ++k) rm = pk_max(rm, Hcol[k]);
+                if constexpr (!REV) {
+                    const uint32_t m = pk_lt_mask(best, rm);
+                    best = pk_max(best, rm);
+                    brow = bfi(m, rowpk, brow);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) snap[k] = bfi(m, Hcol[k], snap[k]);
+                } else {
+                    if (lo16(rm) == target || hi16(rm) == target) {
+                        const int row_lo = s - v_lo, row_hi = s - v_hi;
+                        // first (smallest) column of this row that reaches the target
+                        int k_lo = K, k_hi = K;
+#pragma unroll
+                        for (int k = K - 1; k >= 0; --k) {
+                            if (lo16(Hcol[k]) == target && strip_base + v_lo * K + k < t.nc) k_lo = k;
+                            if (hi16(Hcol[k]) == target && strip_base + v_hi * K + k < t.nc) k_hi = k;
+                        }
+                        if (k_lo < K && row_lo >= 0 && row_lo < t.nr && strip_base + v_lo * K + k_lo < rt_lo) {
+                            rt_lo = strip_base + v_lo * K + k_lo;
+                            rr_lo = row_lo;
+                        }
+                        if (k_hi < K && row_hi >= 0 && row_hi < t.nr && strip_base + v_hi * K + k_hi < rt_hi) {
+                            rt_hi = strip_base + v_hi * K + k_hi;
+                            rr_hi = row_hi;
+                        }
+                    }
+                }
+                rowpk = raw(as_u(rowpk) + (u16x2){1, 1});
+            }
+        }
+
+        // ---- candidates of this strip
+        if constexpr (!REV) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int32_t b = h ? hi16(best) : lo16(best);
+                if (live && b > 0) {
+                    const int32_t row = (int32_t)(h ? (brow >> 16) : (brow & 0xFFFFu));
+                    const int base = strip_base + (h ? v_hi : v_lo) * K;
+                    uint32_t pbest = 0xFFFFFFFFu;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const int32_t v = h ? hi16(snap[k]) : lo16(snap[k]);
+                        const int col = base + k;
+                        if (v == b && col < t.nc) pbest = min(pbest, tie_prio((uint32_t)col));
+                    }
+                    if (pbest != 0xFFFFFFFFu && row < t.nr) {
+                        const uint64_t key = ((uint64_t)b << 40) | ((uint64_t)(0xFFFFFu - (uint32_t)row) << 20) |
+                                             (uint64_t)(0xFFFFFu - pbest);
+                        if (h) cand_hi = key > cand_hi ? key : cand_hi;
+                        else cand_lo = key > cand_lo ? key : cand_lo;
+                    }
+                }
+            }
+        } else {
+            if (live && rt_lo != 0x7FFFFFFF) {
+                const uint64_t key = ((uint64_t)(0xFFFFFu - (uint32_t)rt_lo) << 20) | (uint64_t)(0xFFFFFu - (uint32_t)rr_lo);
+                cand_lo = key > cand_lo ? key : cand_lo;
+            }
+            if (live && rt_hi != 0x7FFFFFFF) {
+                const uint64_t key = ((uint64_t)(0xFFFFFu - (uint32_t)rt_hi) << 20) | (uint64_t)(0xFFFFFu - (uint32_t)rr_hi);
+                cand_hi = key > cand_hi ? key : cand_hi;
+            }
+            // a later strip only holds larger columns: stop once every group has a hit
+            const uint64_t found = __ballot((cand_lo | cand_hi) != 0);
+            bool all_done = true;
+#pragma unroll
+            for (int q = 0; q < kPairsPerWave; ++q) {
+                const bool has = ((found >> (16 * q)) & 0xFFFFull) != 0;
+                const bool more = info[q * 8 + 0] > strip + 1;
+                all_done = all_done && (has || !more);
+            }
+            if (all_done) break;
+        }
+    }
+
+    // ---- reduce the 32 candidates of each group
+    uint64_t* xch = sm.xch();
+    __syncthreads();
+    xch[2 * lane + 0] = cand_lo;
+    xch[2 * lane + 1] = cand_hi;
+    __syncthreads();
+    uint64_t key = 0;
+    for (int q = 0; q < 2 * kLanesPerPair; ++q) {
+        const uint64_t v = xch[g * 2 * kLanesPerPair + q];
+        key = v > key ? v : key;
+    }
+    PassResult r;
+    r.key = key;
+    return r;
+}
+
+struct FwdOut {
+    int32_t score, ref_end, query_end;
+};
+
+__device__ __forceinline__ Task forward_task(const KernelArgs& a, int64_t p)
+{
+    Task t;
+    t.rev = 0;
+    if (p < 0) { t.rows = a.ref; t.cols = a.qry; t.nr = 0; t.nc = 0; return t; }
+    const uint8_t* A = a.ref + a.ref_off[p];
+    const uint8_t* B = a.qry + a.qry_off[p];
+    const int32_t la = (int32_t)(a.ref_off[p + 1] - a.ref_off[p]);
+    const int32_t lb = (int32_t)(a.qry_off[p + 1] - a.qry_off[p]);
+    if (la < lb) { t.rows = B; t.nr = lb; t.cols = A; t.nc = la; } // kernel.cpp:248-253
+    else { t.rows = A; t.nr = la; t.cols = B; t.nc = lb; }
+    return t;
+}
+
+__device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p, uint64_t key, int32_t* status)
+{
+    FwdOut o{0, 0, 0};
+    if (p < 0 || key == 0) return o;
+    const int32_t la = (int32_t)(a.ref_off[p + 1] - a.ref_off[p]);
+    const int32_t lb = (int32_t)(a.qry_off[p + 1] - a.qry_off[p]);
+    const int32_t score = (int32_t)(key >> 40);
+    const int32_t row = (int32_t)(0xFFFFFu - (uint32_t)((key >> 20) & 0xFFFFFu));
+    const int32_t col = (int32_t)inv_prio(0xFFFFFu - (uint32_t)(key & 0xFFFFFu));
+    o.score = score;
+    if (la < lb) { o.query_end = row + 1; o.ref_end = col + 1; } // kernel.cpp:381-386
+    else { o.ref_end = row + 1; o.query_end = col + 1; }        // kernel.cpp:387-392
+    if (score >= a.sat_limit) atomicOr(status + kStatusSaturated, 1);
+    return o;
+}
+
+__device__ __forceinline__ Task reverse_task(const KernelArgs& a, int64_t p, const FwdOut& f)
+{
+    Task t;
+    t.rev = 1;
+    if (p < 0 || f.score <= 0) { t.rows = a.ref; t.cols = a.qry; t.nr = 0; t.nc = 0; return t; }
+    const uint8_t* A = a.ref + a.ref_off[p];
+    const uint8_t* B = a.qry + a.qry_off[p];
+    if (f.ref_end < f.query_end) { t.rows = B; t.nr = f.query_end; t.cols = A; t.nc = f.ref_end; } // 444-451
+    else { t.rows = A; t.nr = f.ref_end; t.cols = B; t.nc = f.query_end; }                    // 452-459
+    return t;
+}
+
+__device__ __forceinline__ void reverse_result(const KernelArgs& a, int64_t p, const FwdOut& f, const Task& t,
+                                               uint64_t key, int32_t* ref_begin, int32_t* query_begin)
+{
+    *ref_begin = 0;
+    *query_begin = 0;
+    if (p < 0 || f.score <= 0) return;
+    if (key == 0) { atomicOr(a.status + kStatusRevMiss, 1); return; }
+    const int32_t col = (int32_t)(0xFFFFFu - (uint32_t)((key >> 20) & 0xFFFFFu));
+    const int32_t row = (int32_t)(0xFFFFFu - (uint32_t)(key & 0xFFFFFu));
+    const int32_t orow = t.nr - 1 - row; // kernel.cpp:579
+    const int32_t ocol = t.nc - 1 - col; // kernel.cpp:580
+    if (f.ref_end < f.query_end) { *query_begin = orow; *ref_begin = ocol; } // 591-595
+    else { *ref_begin = orow; *query_begin = ocol; }                       // 596-600
+}
+
+__device__ __forceinline__ void stage_tables(const KernelArgs& a, const Smem& sm)
+{
+    for (int i = threadIdx.x; i < 256; i += kWave) sm.lut()[i] = a.lut[i];
+    for (int i = threadIdx.x; i < a.alpha * a.alpha; i += kWave) sm.sub()[i] = a.sub[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ int64_t slot_pair(const KernelArgs& a, int64_t slot)
+{
+    if (slot >= a.order_n) return -1;
+    return a.order ? (int64_t)a.order[slot] : slot;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Plain form, forward pass: quads of consecutive pairs in input order, statically
+// strided over the grid (one launch per batch, like driver.cpp:176-185).
+template <int K>
+__global__ __launch_bounds__(64) void k_plain_forward(KernelArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    Smem sm{smem_raw, lds_layout(kVirtPerPair * K, a.alpha)};
+    stage_tables(a, sm);
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
+    const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
+    for (int64_t q = blockIdx.x; q < nquads; q += gridDim.x) {
+        const int64_t p = slot_pair(a, q * kPairsPerWave + g);
+        const Task t = forward_task(a, p);
+        const PassResult r = run_pass<K, false>(a, sm, t, 0, bnd_g);
+        const FwdOut f = forward_result(a, p, r.key, a.status);
+        if (p >= 0 && l == 0) {
+            a.out[kOutScore * a.n + p] = f.score;
+            a.out[kOutRefEnd * a.n + p] = f.ref_end;
+            a.out[kOutQueryEnd * a.n + p] = f.query_end;
+        }
+    }
+}
+
+// Plain form, reverse pass: reads the forward results of the same buffer
+// (driver.cpp:199-208, with no host round trip for the block size).
+template <int K>
+__global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    Smem sm{smem_raw, lds_layout(kVirtPerPair * K, a.alpha)};
+    stage_tables(a, sm);
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
+    const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
+    for (int64_t q = blockIdx.x; q < nquads; q += gridDim.x) {
+        const int64_t p = slot_pair(a, q * kPairsPerWave + g);
+        FwdOut f{0, 0, 0};
+        if (p >= 0) {
+            f.score = a.out[kOutScore * a.n + p];
+            f.ref_end = a.out[kOutRefEnd * a.n + p];
+            f.query_end = a.out[kOutQueryEnd * a.n + p];
+        }
+        const Task t = reverse_task(a, p, f);
+        const PassResult r = run_pass<K, true>(a, sm, t, f.score, bnd_g);
+        int32_t rb, qb;
+        reverse_result(a, p, f, t, r.key, &rb, &qb);
+        if (p >= 0 && l == 0) {
+            a.out[kOutRefBegin * a.n + p] = rb;
+            a.out[kOutQueryBegin * a.n + p] = qb;
+        }
+    }
+}
+
+// Batched form: persistent workgroups pull quads of length-sorted pairs from a
+// device work queue and run forward then reverse back to back.
+template <int K>
+__global__ __launch_bounds__(64) void k_batched(KernelArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    Smem sm{smem_raw, lds_layout(kVirtPerPair * K, a.alpha)};
+    stage_tables(a, sm);
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
+    const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
+    for (;;) {
+        uint32_t q = 0;
+        if (threadIdx.x == 0) q = atomicAdd(a.queue, 1u);
+        q = (uint32_t)__builtin_amdgcn_readfirstlane((int)q);
+        if ((int64_t)q >= nquads) break;
+        const int64_t p = slot_pair(a, (int64_t)q * kPairsPerWave + g);
+        const Task tf = forward_task(a, p);
+        const PassResult rf = run_pass<K, false>(a, sm, tf, 0, bnd_g);
+        const FwdOut f = forward_result(a, p, rf.key, a.status);
+        const Task tr = reverse_task(a, p, f);
+        const PassResult rr = run_pass<K, true>(a, sm, tr, f.score, bnd_g);
+        int32_t rb, qb;
+        reverse_result(a, p, f, tr, rr.key, &rb, &qb);
+        if (p >= 0 && l == 0) {
+            a.out[kOutScore * a.n + p] = f.score;
+            a.out[kOutRefEnd * a.n + p] = f.ref_end;
+            a.out[kOutQueryEnd * a.n + p] = f.query_end;
+            a.out[kOutRefBegin * a.n + p] = rb;
+            a.out[kOutQueryBegin * a.n + p] = qb;
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------- host side
+
+template <int K>
+static const void* kernel_ptr(int kind)
+{
+    switch (kind) {
+    case kPlainForward: return (const void*)&k_plain_forward<K>;
+    case kPlainReverse: return (const void*)&k_plain_reverse<K>;
+    default: return (const void*)&k_batched<K>;
+    }
+}
+
+static const void* kernel_for(int kind, int K)
+{
+    switch (K) {
+    case 2: return kernel_ptr<2>(kind);
+    case 4: return kernel_ptr<4>(kind);
+    case 8: return kernel_ptr<8>(kind);
+    case 16: return kernel_ptr<16>(kind);
+    case 32: return kernel_ptr<32>(kind);
+    default: return nullptr;
+    }
+}
+
+bool kernel_supported(int K) { return K == 2 || K == 4 || K == 8 || K == 16 || K == 32; }
+
+// Workgroups of one kernel that fit on a CU at this LDS size (0 on error).
+int kernel_occupancy(int kind, int K, int lds_bytes)
+{
+    const void* f = kernel_for(kind, K);
+    if (!f) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kWave, (size_t)lds_bytes) != hipSuccess) return 0;
+    return nb;
+}
+
+hipError_t kernel_launch(int kind, int K, int grid, int lds_bytes, hipStream_t stream, const KernelArgs& args)
+{
+    const void* f = kernel_for(kind, K);
+    if (!f) return hipErrorInvalidValue;
+    if (lds_bytes > 65536) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    KernelArgs a = args;
+    void* params[] = {&a};
+    return hipLaunchKernel(f, dim3(grid), dim3(kWave), params, (size_t)lds_bytes, stream);
+}
+
+hipError_t zero_u32(uint32_t* p, int n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+} // namespace gbsw

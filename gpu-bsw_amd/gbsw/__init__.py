@@ -1,0 +1,239 @@
+"""Python host binding of the MI355X batched Smith-Waterman engine (libgbsw.so).
+
+Mirrors the reference's driver entry points (driver.hpp:32-37):
+``kernel_driver_dna(reads, contigs, scores)`` and
+``kernel_driver_aa(reads, contigs, scoring_matrix, open_gap, extend_gap)`` return
+the five ``alignment_results`` fields (driver.hpp:21-27) as int32 numpy arrays.
+Everything goes through the C ABI in include/gbsw.h; there is no fallback: if
+the shared library is missing or no GPU is present, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .scoring import AA_ORDER, BLOSUM62, encode_aa  # noqa: F401
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT_DIR = os.path.dirname(PKG_DIR)  # gpu-bsw_amd/
+LIB_PATH = os.path.join(ROOT_DIR, "libgbsw.so")
+
+DNA, PROTEIN = 0, 1
+PLAIN, BATCHED = 0, 1
+MODES = {"plain": PLAIN, "batched": BATCHED}
+FIELDS = ("score", "ref_begin", "ref_end", "query_begin", "query_end")
+
+GBSW_OK = 0
+ERRORS = {-1: "ARGUMENT", -2: "RESIDUE", -3: "SCORING", -4: "DEVICE", -5: "MEMORY", -6: "RANGE", -7: "INTERNAL"}
+
+
+class GbswError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gbsw error %d (%s): %s" % (code, ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+class Scoring(ctypes.Structure):
+    _fields_ = [
+        ("alphabet", ctypes.c_int32),
+        ("match", ctypes.c_int16),
+        ("mismatch", ctypes.c_int16),
+        ("gap_open", ctypes.c_int16),
+        ("gap_extend", ctypes.c_int16),
+        ("matrix", ctypes.c_int16 * 576),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("device_ms", ctypes.c_double),
+        ("main_kernel_ms", ctypes.c_double),
+        ("cells", ctypes.c_int64),
+        ("launches", ctypes.c_int32),
+        ("buckets", ctypes.c_int32),
+    ]
+
+
+# Every symbol include/gbsw.h declares (checked by the CPU test suite).
+EXPORTED = (
+    "gbsw_context_create", "gbsw_context_destroy", "gbsw_last_error", "gbsw_batch_create",
+    "gbsw_batch_destroy", "gbsw_batch_align", "gbsw_batch_results", "gbsw_batch_device_results",
+    "gbsw_batch_stats", "gbsw_align", "gbsw_device_memory", "gbsw_version",
+)
+
+_lib = None
+
+
+def library_path():
+    return os.environ.get("GBSW_LIBRARY", LIB_PATH)
+
+
+def load_library():
+    """Load libgbsw.so (built in-tree by __graft_entry__.build() / make -C gpu-bsw_amd)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise FileNotFoundError("libgbsw.so not built: %s (run `make -C gpu-bsw_amd`)" % path)
+    lib = ctypes.CDLL(path)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    lib.gbsw_context_create.argtypes = [i32, ctypes.POINTER(vp)]
+    lib.gbsw_context_destroy.argtypes = [vp]
+    lib.gbsw_last_error.restype = ctypes.c_char_p
+    lib.gbsw_version.restype = ctypes.c_char_p
+    lib.gbsw_batch_create.argtypes = [vp, i32, i64, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    lib.gbsw_batch_destroy.argtypes = [vp]
+    lib.gbsw_batch_align.argtypes = [vp, ctypes.POINTER(Scoring), i32]
+    lib.gbsw_batch_results.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.gbsw_batch_device_results.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.gbsw_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    lib.gbsw_align.argtypes = [vp, ctypes.POINTER(Scoring), i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.gbsw_device_memory.argtypes = [i32, ctypes.POINTER(ctypes.c_uint64)]
+    for name in EXPORTED:
+        if name not in ("gbsw_last_error", "gbsw_version"):
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != GBSW_OK:
+        raise GbswError(rc, load_library().gbsw_last_error().decode(errors="replace"))
+
+
+def dna_scoring(match=3, mismatch=-3, gap_open=-6, gap_extend=-1):
+    sc = Scoring()
+    sc.alphabet, sc.match, sc.mismatch, sc.gap_open, sc.gap_extend = DNA, match, mismatch, gap_open, gap_extend
+    return sc
+
+
+def protein_scoring(matrix=None, gap_open=-6, gap_extend=-1):
+    sc = Scoring()
+    sc.alphabet, sc.gap_open, sc.gap_extend = PROTEIN, gap_open, gap_extend
+    m = BLOSUM62 if matrix is None else list(matrix)
+    if len(m) != 576:
+        raise ValueError("protein scoring matrix needs 576 entries")
+    for k, v in enumerate(m):
+        sc.matrix[k] = int(v)
+    return sc
+
+
+def pack(seqs):
+    """List of str/bytes -> (uint8 buffer, uint64 offsets of n+1)."""
+    bs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    off = np.zeros(len(bs) + 1, dtype=np.uint64)
+    if bs:
+        off[1:] = np.cumsum([len(b) for b in bs], dtype=np.uint64)
+    buf = np.frombuffer(b"".join(bs), dtype=np.uint8) if off[-1] else np.zeros(1, dtype=np.uint8)
+    return np.ascontiguousarray(buf), off
+
+
+class Context:
+    """One HIP device, one stream (gbsw_context)."""
+
+    def __init__(self, device=0):
+        self._lib = load_library()
+        self._h = ctypes.c_void_p()
+        _check(self._lib.gbsw_context_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            self._lib.gbsw_context_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def batch(self, alphabet, ref, ref_off, query, query_off):
+        return Batch(self, alphabet, ref, ref_off, query, query_off)
+
+    def align(self, refs, queries, scoring, mode="batched"):
+        """Host lists in, dict of five int32 arrays out (gbsw_align)."""
+        rb, ro = pack(refs)
+        qb, qo = pack(queries)
+        return self.align_packed(rb, ro, qb, qo, scoring, mode)
+
+    def align_packed(self, rb, ro, qb, qo, scoring, mode="batched"):
+        n = len(ro) - 1
+        if len(qo) - 1 != n:
+            raise ValueError("ref and query counts differ")
+        out = np.zeros((5, max(n, 1)), dtype=np.int32)
+        ptrs = [out[f].ctypes.data for f in range(5)]
+        rb = np.ascontiguousarray(rb, dtype=np.uint8)
+        qb = np.ascontiguousarray(qb, dtype=np.uint8)
+        ro = np.ascontiguousarray(ro, dtype=np.uint64)
+        qo = np.ascontiguousarray(qo, dtype=np.uint64)
+        _check(self._lib.gbsw_align(self._h, ctypes.byref(scoring), MODES.get(mode, mode), n, rb.ctypes.data,
+                                    ro.ctypes.data, qb.ctypes.data, qo.ctypes.data, *ptrs))
+        return {f: out[k][:n].copy() for k, f in enumerate(FIELDS)}
+
+
+class Batch:
+    """A batch resident in device memory (gbsw_batch)."""
+
+    def __init__(self, ctx, alphabet, ref, ref_off, query, query_off):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        self._h = ctypes.c_void_p()
+        self._keep = [np.ascontiguousarray(x) for x in (ref, ref_off, query, query_off)]
+        rb, ro, qb, qo = self._keep
+        assert ro.dtype == np.uint64 and qo.dtype == np.uint64
+        self.n = len(ro) - 1
+        _check(self._lib.gbsw_batch_create(ctx._h, alphabet, self.n, rb.ctypes.data, ro.ctypes.data,
+                                           qb.ctypes.data, qo.ctypes.data, ctypes.byref(self._h)))
+        self._keep = None
+
+    def align(self, scoring, mode="batched"):
+        _check(self._lib.gbsw_batch_align(self._h, ctypes.byref(scoring), MODES.get(mode, mode)))
+
+    def results(self):
+        out = np.zeros((5, max(self.n, 1)), dtype=np.int32)
+        _check(self._lib.gbsw_batch_results(self._h, *[out[f].ctypes.data for f in range(5)]))
+        return {f: out[k][:self.n].copy() for k, f in enumerate(FIELDS)}
+
+    def stats(self):
+        st = Stats()
+        _check(self._lib.gbsw_batch_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def close(self):
+        if self._h:
+            self._lib.gbsw_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("GBSW_DEVICE", "0")))
+    return _default_ctx
+
+
+def kernel_driver_dna(reads, contigs, scores, factor=1.0, mode="batched"):
+    """driver.hpp:32-33.  reads[k] (query) is aligned with contigs[k] (ref);
+    scores = (match, mismatch, gap_open, gap_extend)."""
+    del factor  # memory fraction of the reference's chunking; batches are resident whole
+    n = len(contigs)
+    sc = dna_scoring(*[int(v) for v in scores])
+    return default_context().align(list(contigs), list(reads)[:n], sc, mode)
+
+
+def kernel_driver_aa(reads, contigs, scoring_matrix, open_gap, extend_gap, factor=1.0, mode="batched"):
+    """driver.hpp:36-37 with the 24x24 matrix in ARNDCQEGHILKMFPSTWYVBZX* order."""
+    del factor
+    n = len(contigs)
+    sc = protein_scoring(scoring_matrix, int(open_gap), int(extend_gap))
+    return default_context().align(list(contigs), list(reads)[:n], sc, mode)

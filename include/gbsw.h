@@ -1,0 +1,117 @@
+/*
+ * gbsw.h -- C ABI of the MI355X batched Smith-Waterman engine (libgbsw.so).
+ *
+ * This is the boundary a host program binds to (ctypes, cgo, JNI, N-API ...):
+ * plain pointers and sizes, status codes, no C++ or framework types.  The C++
+ * drop-in gpu_bsw_driver::kernel_driver_dna / kernel_driver_aa
+ * (include/gpu_bsw_driver.hpp) is built on these calls.
+ *
+ * Semantics follow the reference exactly (GPU-BSW / ADEPT):
+ *   pair k aligns ref k (the reference's "contig", seqA) with query k (the
+ *   reference's "read", seqB), driver.cpp:16-21 / 246-250;
+ *   outputs per pair: top score, 0-based begins, 1-based ends, with the
+ *   reference's orientation and tie rules (kernel.cpp:198-1059);
+ *   DNA scores compare raw bytes (kernel.cpp:366); protein residues are encoded
+ *   with the reference's 91-entry table (driver.cpp:252-258) and scored with a
+ *   caller-supplied 24x24 matrix indexed enc(row)*24 + enc(col) (kernel.cpp:785-788).
+ * Differences, by design: results are int32; errors are returned as status
+ * codes instead of exit() (utils.hpp:5-21); one device per context.
+ */
+#ifndef GBSW_H
+#define GBSW_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GBSW_OK 0
+#define GBSW_E_ARGUMENT (-1)  /* null pointer, bad size, bad mode */
+#define GBSW_E_RESIDUE (-2)   /* protein byte >= 91 or negative; DNA alphabet > 31 symbols */
+#define GBSW_E_SCORING (-3)   /* gap scores must be <= 0 */
+#define GBSW_E_DEVICE (-4)    /* HIP runtime error (message in gbsw_last_error) */
+#define GBSW_E_MEMORY (-5)    /* device or pinned allocation failed */
+#define GBSW_E_RANGE (-6)     /* scores or lengths beyond the engine's exact range */
+#define GBSW_E_INTERNAL (-7)  /* an internal consistency check failed */
+
+#define GBSW_DNA 0
+#define GBSW_PROTEIN 1
+
+/* Execution forms.
+ * PLAIN:   the reference's shape -- one forward launch and one reverse launch per
+ *          batch, pairs in input order, one kernel geometry sized for the longest
+ *          pair of the batch (driver.cpp:169-208).
+ * BATCHED: pairs binned by length, one persistent kernel per length bucket pulling
+ *          work from a device queue, forward and reverse fused, geometry chosen
+ *          per bucket. */
+#define GBSW_PLAIN 0
+#define GBSW_BATCHED 1
+
+typedef struct gbsw_context gbsw_context;
+typedef struct gbsw_batch gbsw_batch;
+
+typedef struct {
+    int32_t alphabet;      /* GBSW_DNA or GBSW_PROTEIN */
+    int16_t match;         /* DNA: score of equal bytes   (reference scores[0]) */
+    int16_t mismatch;      /* DNA: score of unequal bytes (scores[1])           */
+    int16_t gap_open;      /* first gap position, <= 0    (scores[2] / openGap)  */
+    int16_t gap_extend;    /* each further position, <= 0 (scores[3] / extendGap)*/
+    int16_t matrix[576];   /* protein: 24x24, ARNDCQEGHILKMFPSTWYVBZX* order     */
+} gbsw_scoring;
+
+typedef struct {
+    double device_ms;       /* last gbsw_batch_align: device time, hipEvents        */
+    double main_kernel_ms;  /* time of the dominant kernel launch(es)               */
+    int64_t cells;          /* sum of |ref|*|query| (the reference's "Total Cells") */
+    int32_t launches;       /* kernel launches issued                               */
+    int32_t buckets;        /* length buckets used (BATCHED)                        */
+} gbsw_stats;
+
+/* Create a context on one HIP device.  A context owns a stream, scratch buffers
+ * and events; calls on one context must not overlap. */
+int gbsw_context_create(int32_t device, gbsw_context** out);
+int gbsw_context_destroy(gbsw_context* ctx);
+
+/* Message of the last error on this thread ("" if none). */
+const char* gbsw_last_error(void);
+
+/* Upload a batch into device memory.  Pair k is
+ *   ref[ref_off[k] .. ref_off[k+1])  against  query[query_off[k] .. query_off[k+1])
+ * (n+1 offsets each, non-decreasing).  Residues are validated for the alphabet
+ * here.  Host buffers may be reused after the call returns. */
+int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char* ref,
+                      const uint64_t* ref_off, const char* query, const uint64_t* query_off,
+                      gbsw_batch** out);
+int gbsw_batch_destroy(gbsw_batch* batch);
+
+/* Align every pair of a resident batch; results stay in device memory. */
+int gbsw_batch_align(gbsw_batch* batch, const gbsw_scoring* scoring, int32_t mode);
+
+/* Copy the results of the last gbsw_batch_align to host arrays of n int32 each
+ * (any pointer may be NULL to skip that field). */
+int gbsw_batch_results(gbsw_batch* batch, int32_t* score, int32_t* ref_begin, int32_t* ref_end,
+                       int32_t* query_begin, int32_t* query_end);
+
+/* Device pointer of the five result arrays (score, ref_begin, ref_end,
+ * query_begin, query_end; n int32 each, back to back). */
+int gbsw_batch_device_results(gbsw_batch* batch, const int32_t** d_out);
+
+int gbsw_batch_stats(const gbsw_batch* batch, gbsw_stats* out);
+
+/* One call, host memory in and out: upload, align, download (the drop-in path). */
+int gbsw_align(gbsw_context* ctx, const gbsw_scoring* scoring, int32_t mode, int64_t n,
+               const char* ref, const uint64_t* ref_off, const char* query,
+               const uint64_t* query_off, int32_t* score, int32_t* ref_begin, int32_t* ref_end,
+               int32_t* query_begin, int32_t* query_end);
+
+/* Total device memory in bytes (driver.cpp:10-14, get_tot_gpu_mem). */
+int gbsw_device_memory(int32_t device, uint64_t* bytes);
+
+/* Version string of the library. */
+const char* gbsw_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

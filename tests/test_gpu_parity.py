@@ -1,0 +1,194 @@
+"""GPU parity: the HIP engine through the C ABI against the CPU oracle.
+
+Bar: bit-exact on all five outputs (score, ref_begin, ref_end, query_begin,
+query_end) for every pair, in both execution forms (plain and batched).
+"""
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import gbsw
+from gbsw import synth
+from gbsw.scoring import BLOSUM62
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIELDS = gbsw.FIELDS
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = gbsw.Context(0)
+    yield c
+    c.close()
+
+
+def oracle_scoring(alphabet, scoring):
+    if alphabet == gbsw.DNA:
+        return ol.dna_scoring(*scoring)
+    return ol.aa_scoring(BLOSUM62, *scoring)
+
+
+def gbsw_scoring(alphabet, scoring):
+    if alphabet == gbsw.DNA:
+        return gbsw.dna_scoring(*scoring)
+    return gbsw.protein_scoring(BLOSUM62, *scoring)
+
+
+def oracle_results(rb, ro, qb, qo, alphabet, scoring):
+    out = ol.align_batch(rb, ro, qb, qo, oracle_scoring(alphabet, scoring))
+    return {f: out[f] for f in FIELDS}
+
+
+def assert_same(got, exp, label, refs=None):
+    for f in FIELDS:
+        bad = np.nonzero(got[f] != exp[f])[0]
+        if len(bad):
+            k = int(bad[0])
+            detail = {g: (int(got[g][k]), int(exp[g][k])) for g in FIELDS}
+            raise AssertionError("%s: %d pairs differ in %s; first pair %d (got, want) %s" %
+                                 (label, len(bad), f, k, detail))
+
+
+def run_both(ctx, rb, ro, qb, qo, alphabet, scoring, label):
+    exp = oracle_results(rb, ro, qb, qo, alphabet, scoring)
+    for mode in ("plain", "batched"):
+        print("case", label, mode, "pairs", len(ro) - 1, flush=True)
+        got = ctx.align_packed(rb, ro, qb, qo, gbsw_scoring(alphabet, scoring), mode)
+        assert_same(got, exp, "%s/%s" % (label, mode))
+    return exp
+
+
+def rand_pairs(seed, n, lo, hi, alphabets, query_lo=None, query_hi=None):
+    rng = random.Random(seed)
+    refs, qrys = [], []
+    for _ in range(n):
+        alpha = rng.choice(alphabets)
+        refs.append("".join(rng.choice(alpha) for _ in range(rng.randint(lo, hi))))
+        qrys.append("".join(rng.choice(alpha) for _ in range(rng.randint(query_lo or lo, query_hi or hi))))
+    return refs, qrys
+
+
+def test_fixtures(ctx):
+    import json
+    with open(os.path.join(HERE, "golden", "tie_fixtures.json")) as fh:
+        fx = json.load(fh)
+    refs = [c[0] for c in fx["cases"]]
+    qrys = [c[1] for c in fx["cases"]]
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    exp = run_both(ctx, rb, ro, qb, qo, gbsw.DNA, tuple(fx["scoring"]), "tie_fixtures")
+    for k, c in enumerate(fx["cases"]):
+        assert tuple(int(exp[f][k]) for f in FIELDS) == tuple(c[2])
+
+
+@pytest.mark.parametrize("scheme", [(1, -3, -3, -1), (3, -3, -6, -1), (2, -1, -2, -2), (5, -4, -10, 0)])
+def test_random_dna_small_alphabets(ctx, scheme):
+    refs, qrys = rand_pairs(hash(scheme) & 0xFFFF, 600, 0, 90, ["AC", "ACG", "ACGT", "AAC"])
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    run_both(ctx, rb, ro, qb, qo, gbsw.DNA, scheme, "dna_small_%s" % (scheme,))
+
+
+def test_random_protein_small(ctx):
+    refs, qrys = rand_pairs(77, 500, 0, 120, ["ARND", "WYFHA", "ARNDCQEGHILKMFPSTWYV", "BZX*AC"])
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    run_both(ctx, rb, ro, qb, qo, gbsw.PROTEIN, (-6, -1), "protein_small")
+
+
+def test_tie_heavy_cases(ctx):
+    from test_oracle import _tie_heavy_cases
+    cases = _tie_heavy_cases(31, 400) + _tie_heavy_cases(32, 400)
+    rb, ro = gbsw.pack([c[0] for c in cases])
+    qb, qo = gbsw.pack([c[1] for c in cases])
+    run_both(ctx, rb, ro, qb, qo, gbsw.DNA, (1, -3, -3, -1), "tie_heavy")
+
+
+def test_edge_cases(ctx):
+    refs = ["", "A", "", "A", "C", "ACGT", "NNNN", "acgtACGT", "AAAA", "ACGTN" * 40, "A" * 700, "T"]
+    qrys = ["", "", "A", "A", "A", "ACGT", "NNAN", "ACGTacgt", "CCCC", "ACGT" * 60, "A" * 700, "A" * 1500]
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    exp = run_both(ctx, rb, ro, qb, qo, gbsw.DNA, (1, -3, -3, -1), "edges")
+    assert exp["score"][0] == 0 and exp["score"][3] == 1 and exp["score"][8] == 0
+
+
+def test_single_pair_and_odd_counts(ctx):
+    """Batches of 1, 2, 3 and 5 pairs (the reference needs >= 2 per device)."""
+    refs, qrys = rand_pairs(5, 5, 20, 60, ["ACGT"])
+    for n in (1, 2, 3, 5):
+        rb, ro = gbsw.pack(refs[:n])
+        qb, qo = gbsw.pack(qrys[:n])
+        run_both(ctx, rb, ro, qb, qo, gbsw.DNA, (3, -3, -6, -1), "n=%d" % n)
+
+
+def test_multi_strip_dna(ctx):
+    """Shorter sequences longer than one strip: boundary columns pass through
+    global memory between strips (plain K=32 covers 1024 columns)."""
+    refs, qrys = rand_pairs(9, 40, 1100, 2600, ["ACGT", "AC"])
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    run_both(ctx, rb, ro, qb, qo, gbsw.DNA, (3, -3, -6, -1), "multi_strip_dna")
+
+
+def test_multi_strip_protein_mutated(ctx):
+    """Long-protein generator, cut to refs <= 3000 and queries <= 1500 so every
+    score is provably inside int16, crossing up to six 256-column strips."""
+    w = synth.long_protein(24)
+    refs = [bytes(w.pair(k)[0][:3000]) for k in range(w.n)]
+    qrys = [bytes(w.pair(k)[1][:1500]) for k in range(w.n)]
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    run_both(ctx, rb, ro, qb, qo, w.alphabet, w.scoring, "long_protein_24_cut")
+
+
+def test_config0_plain_dna_full(ctx):
+    w = synth.plain_dna()
+    run_both(ctx, w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring, "config0_full")
+
+
+def test_config1_plain_protein_full(ctx):
+    w = synth.plain_protein()
+    run_both(ctx, w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring, "config1_full")
+
+
+def test_config2_batched_dna_subset(ctx):
+    w = synth.batched_dna(20_000, seed=123)
+    run_both(ctx, w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring, "config2_20k")
+
+
+def test_resident_batch_reuse(ctx):
+    """One resident batch aligned twice with different scorings and modes."""
+    w = synth.plain_dna(3000, seed=4)
+    b = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
+    for scheme in [(3, -3, -6, -1), (1, -3, -3, -1)]:
+        exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, 0, scheme)
+        for mode in ("batched", "plain"):
+            b.align(gbsw.dna_scoring(*scheme), mode)
+            assert_same(b.results(), exp, "reuse/%s/%s" % (scheme, mode))
+            st = b.stats()
+            assert st["cells"] == w.cells() and st["device_ms"] > 0
+    b.close()
+
+
+def test_errors_are_reported(ctx):
+    with pytest.raises(gbsw.GbswError) as e:
+        ctx.align(["ACD"], ["ACa"], gbsw.protein_scoring())  # 'a' = 97 >= 91
+    assert e.value.code == -2
+    with pytest.raises(gbsw.GbswError) as e:
+        ctx.align(["ACGT"], ["ACGT"], gbsw.dna_scoring(1, -1, 2, -1))  # positive gap score
+    assert e.value.code == -3
+
+
+def test_drop_in_python_driver(ctx):
+    reads, contigs = ["ACGTTACGT", "ACGT"], ["GGGGGACGTGGGG", "TTACGTTT"]
+    r = gbsw.kernel_driver_dna(reads, contigs, [1, -3, -3, -1])
+    assert (int(r["score"][0]), int(r["ref_begin"][0]), int(r["ref_end"][0]),
+            int(r["query_begin"][0]), int(r["query_end"][0])) == (4, 5, 9, 5, 9)
+    a = gbsw.kernel_driver_aa(["MKV"], ["AMKVA"], BLOSUM62, -6, -1)
+    assert int(a["score"][0]) == 5 + 5 + 4
