@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -78,7 +79,7 @@ struct gbsw_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> kev; // per-launch event pairs
-    DeviceBuffer bnd, queue, status, sub, lut;
+    DeviceBuffer bnd, queue, status, sub, lut, colmax;
 };
 
 struct gbsw_batch {
@@ -100,22 +101,37 @@ struct gbsw_batch {
 
 namespace {
 
-int lds_for(int K, int alpha) { return lds_layout(kVirtPerPair * K, alpha).total; }
+int lds_for(int K, int alpha) { return lds_layout(kVirtPerPair * kstride(K), alpha).total; }
 
-// Largest K whose LDS footprint keeps at least two workgroups per CU.
+// Largest compiled K whose LDS footprint fits a 64 KiB workgroup.
 int max_k_for(int alpha)
 {
-    int best = 2;
-    for (int K : {2, 4, 8, 16, 32})
-        if (lds_for(K, alpha) <= 65536) best = K;
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    int best = ks[0];
+    for (int i = 0; i < nk; ++i)
+        if (lds_for(ks[i], alpha) <= 65536) best = ks[i];
     return best;
 }
 
+// Geometry for a shorter sequence of `cols` residues: the K (columns per
+// virtual lane) minimising strips x (K + 2), i.e. padded columns plus the fixed
+// per-step cost of a step (about two columns' worth of work); ties go to fewer
+// strips.
 int k_for_cols(int cols, int kmax)
 {
-    for (int K : {2, 4, 8, 16, 32})
-        if (K <= kmax && kVirtPerPair * K >= cols) return K;
-    return kmax;
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    int best = ks[0];
+    int64_t best_cost = INT64_MAX;
+    for (int i = 0; i < nk; ++i) {
+        const int K = ks[i];
+        if (K > kmax) break;
+        const int64_t strips = std::max<int64_t>(1, (cols + kVirtPerPair * K - 1) / (kVirtPerPair * K));
+        const int64_t cost = strips * (K + 2) * 4 + (strips - 1);
+        if (cost < best_cost) { best_cost = cost; best = K; }
+    }
+    return best;
 }
 
 int grid_for(gbsw_context* ctx, int kind, int K, int lds, int64_t nquads)
@@ -208,7 +224,9 @@ KernelArgs base_args(gbsw_batch* b, const gbsw_scoring* sc, int max_sub)
     a.status = ctx->status.as<int32_t>();
     a.lut = ctx->lut.as<uint8_t>();
     a.sub = ctx->sub.as<int16_t>();
+    a.colmax = ctx->colmax.as<int16_t>();
     a.alpha = b->alpha;
+    a.gap_min = std::min(-(int32_t)sc->gap_open, -(int32_t)sc->gap_extend);
     a.gap_open = sc->gap_open;
     a.gap_extend = sc->gap_extend;
     a.sat_limit = 32767 - max_sub;
@@ -269,6 +287,7 @@ int gbsw_context_create(int32_t device, gbsw_context** out)
     if (e == hipSuccess) e = ctx->queue.reserve(sizeof(uint32_t) * 64);
     if (e == hipSuccess) e = ctx->sub.reserve(sizeof(int16_t) * 1024);
     if (e == hipSuccess) e = ctx->lut.reserve(256);
+    if (e == hipSuccess) e = ctx->colmax.reserve(sizeof(int16_t) * 32);
     if (e != hipSuccess) {
         gbsw_context_destroy(ctx);
         return fail(GBSW_E_DEVICE, std::string("context setup: ") + hipGetErrorString(e));
@@ -287,6 +306,7 @@ int gbsw_context_destroy(gbsw_context* ctx)
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     ctx->bnd.release(); ctx->queue.release(); ctx->status.release(); ctx->sub.release(); ctx->lut.release();
+    ctx->colmax.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GBSW_OK;
@@ -411,6 +431,12 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
 
     HIP_TRY(hipMemcpyAsync(ctx->sub.ptr, sub.data(), sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->lut.ptr, b->lut, 256, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<int16_t> colmax((size_t)b->alpha, 0);
+    for (int c = 0; c < b->alpha; ++c)
+        for (int r = 0; r < b->alpha; ++r)
+            colmax[(size_t)c] = std::max<int16_t>(colmax[(size_t)c], sub[(size_t)(r * b->alpha + c)]);
+    HIP_TRY(hipMemcpyAsync(ctx->colmax.ptr, colmax.data(), sizeof(int16_t) * colmax.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
     KernelArgs a = base_args(b, sc, max_sub);
     const int kmax = max_k_for(b->alpha);
     std::vector<std::pair<int, int>> timed; // (event slot, is main kernel)

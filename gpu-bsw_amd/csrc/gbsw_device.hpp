@@ -32,17 +32,21 @@ __device__ __forceinline__ uint32_t pk_sub_floor0(uint32_t a, uint32_t b)
 {
     return raw(__builtin_elementwise_sub_sat(as_u(a), as_u(b)));
 }
-// 0xFFFF in each half where a < b (signed), else 0.
+// 0xFFFF in each half where a < b (signed; |a - b| < 2^15), else 0:
+// v_pk_sub_i16 + v_pk_ashrrev_i16.  Written as asm because the compiler
+// otherwise expands the sign test into compares, selects and a permute.
 __device__ __forceinline__ uint32_t pk_lt_mask(uint32_t a, uint32_t b)
 {
-    return raw((as_s(a) - as_s(b)) >> (s16x2){15, 15});
+    uint32_t d, m;
+    asm("v_pk_sub_i16 %0, %1, %2" : "=v"(d) : "v"(a), "v"(b));
+    asm("v_pk_ashrrev_i16 %0, 15, %1 op_sel_hi:[0,1]" : "=v"(m) : "v"(d));
+    return m;
 }
-// v_bfi_b32: (m & a) | (~m & b).  Written as asm because the compiler otherwise
-// rebuilds a sign-derived select out of compares and SDWA moves.
+// v_bfi_b32: (m & a) | (~m & b).  Written as asm for the same reason.
 __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b)
 {
     uint32_t r;
-    asm volatile("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
     return r;
 }
 // Rotate right by one lane inside each 16-lane row (DPP row_ror:1).

@@ -34,26 +34,34 @@ struct KernelArgs {
     int32_t* status;
     const uint8_t* lut;        // byte -> residue code, 0xFF for bytes outside the alphabet
     const int16_t* sub;        // A*A scores, index row_code*A + col_code
+    const int16_t* colmax;     // per column code: max(0, max_a sub[a*A + code])
     int32_t alpha;             // A
+    int32_t gap_min;           // min(-gap_open, -gap_extend): cost floor of one gap position
     int32_t gap_open;          // <= 0
     int32_t gap_extend;        // <= 0
     int32_t sat_limit;         // scores >= this may have saturated int16
 };
 
-// Dynamic LDS layout of one 64-lane workgroup for a strip width W.
+// Dynamic LDS layout of one 64-lane workgroup; W = profile columns per row
+// (32 virtual lanes x kstride(K)).
 struct LdsLayout {
-    int32_t lut_off, sub_off, xch_off, info_off, group_off, group_bytes;
+    int32_t lut_off, sub_off, colmax_off, xch_off, info_off, group_off, group_bytes;
     int32_t prof_bytes, ring_code_off, ring_bnd_off, total;
 };
 
 __host__ __device__ inline int32_t align_up(int32_t x, int32_t a) { return (x + a - 1) / a * a; }
+
+// Profile columns reserved per virtual lane for K columns: a whole number of
+// LDS read units (b32, b64 or b128 multiples), so every read is aligned.
+__host__ __device__ constexpr int32_t kstride(int32_t K) { return K <= 2 ? 2 : K <= 4 ? 4 : (K + 7) / 8 * 8; }
 
 __host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha)
 {
     LdsLayout L;
     L.lut_off = 0;                                       // 256 B
     L.sub_off = 256;                                     // kMaxAlpha^2 int16
-    L.xch_off = align_up(L.sub_off + 2 * 32 * 32, 16);   // 64 lanes x 2 keys x 8 B
+    L.colmax_off = L.sub_off + 2 * 32 * 32;              // 32 int16
+    L.xch_off = align_up(L.colmax_off + 2 * 32, 16);     // 64 lanes x 2 keys x 8 B
     L.info_off = L.xch_off + 64 * 2 * 8;                 // 4 groups x 8 int32
     L.group_off = align_up(L.info_off + 4 * 8 * 4, 128);
     L.prof_bytes = align_up((alpha + 1) * W * 2, 128);

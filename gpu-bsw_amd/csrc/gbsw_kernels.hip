@@ -38,10 +38,11 @@ namespace gbsw {
 
 // One pass (forward or reverse) of one pair, identical in the 16 lanes of a group.
 struct Task {
-    const uint8_t* rows;  // row sequence (the longer one of this pass)
-    const uint8_t* cols;  // column sequence
-    int32_t nr, nc;
-    int32_t rev;          // 1: both sequences are read backwards (reverse pass)
+    const uint8_t* rows;      // row sequence (the longer one of this pass)
+    const uint8_t* cols;      // column sequence
+    int32_t nr, nc;           // rows / columns computed
+    int32_t nr_full, nc_full; // extents the reversed indices count back from
+    int32_t rev;              // 1: both sequences are read backwards (reverse pass)
 };
 
 struct Smem {
@@ -49,6 +50,7 @@ struct Smem {
     LdsLayout L;
     __device__ uint8_t* lut() const { return base + L.lut_off; }
     __device__ int16_t* sub() const { return (int16_t*)(base + L.sub_off); }
+    __device__ int16_t* colmax() const { return (int16_t*)(base + L.colmax_off); }
     __device__ uint64_t* xch() const { return (uint64_t*)(base + L.xch_off); }
     __device__ int32_t* info() const { return (int32_t*)(base + L.info_off); }
     __device__ uint8_t* group(int g) const { return base + L.group_off + g * L.group_bytes; }
@@ -78,8 +80,8 @@ __device__ __forceinline__ RowFetch row_fetch(const Task& t, int chunk, int l, c
     RowFetch f;
     const int x0 = chunk * kChunk + l, x1 = x0 + 16;
     const bool v0 = x0 >= 0 && x0 < t.nr, v1 = x1 >= 0 && x1 < t.nr;
-    f.b0 = v0 ? (int32_t)t.rows[t.rev ? t.nr - 1 - x0 : x0] : -1;
-    f.b1 = v1 ? (int32_t)t.rows[t.rev ? t.nr - 1 - x1 : x1] : -1;
+    f.b0 = v0 ? (int32_t)t.rows[t.rev ? t.nr_full - 1 - x0 : x0] : -1;
+    f.b1 = v1 ? (int32_t)t.rows[t.rev ? t.nr_full - 1 - x1 : x1] : -1;
     f.d0 = (v0 && use_bnd) ? load_coherent(bnd + x0) : 0u;
     f.d1 = (v1 && use_bnd) ? load_coherent(bnd + x1) : 0u;
     return f;
@@ -99,23 +101,38 @@ __device__ __forceinline__ void row_store(const RowFetch& f, int chunk, int l, u
     ring_bnd[x1 & (kRing - 1)] = f.d1;
 }
 
-// Load the K profile scores of one virtual lane: K int16 = K/2 dwords.
-template <int K>
-__device__ __forceinline__ void prof_load(const uint8_t* p, uint32_t (&d)[K / 2])
+// Load the KS = kstride(K) profile scores reserved for one virtual lane
+// (KS/2 dwords; only the first K are used).
+template <int KS>
+__device__ __forceinline__ void prof_load(const uint8_t* p, uint32_t (&d)[KS / 2])
 {
-    if constexpr (K % 8 == 0) {
+    if constexpr (KS % 8 == 0) {
 #pragma unroll
-        for (int q = 0; q < K / 8; ++q) {
+        for (int q = 0; q < KS / 8; ++q) {
             const uint4 v = *(const uint4*)(p + 16 * q);
             d[4 * q + 0] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
         }
-    } else if constexpr (K == 4) {
+    } else if constexpr (KS == 4) {
         const uint2 v = *(const uint2*)p;
         d[0] = v.x; d[1] = v.y;
     } else {
-        static_assert(K == 2, "K must be 2, 4 or a multiple of 8");
+        static_assert(KS == 2, "profile stride must be 2, 4 or a multiple of 8");
         d[0] = *(const uint32_t*)p;
     }
+}
+
+// Max over K registers as a balanced tree (depth log2 K instead of a chain).
+template <int K>
+__device__ __forceinline__ uint32_t tree_max(const uint32_t (&h)[K])
+{
+    uint32_t t[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] = h[k];
+#pragma unroll
+    for (int w = 1; w < K; w *= 2)
+#pragma unroll
+        for (int k = 0; k + w < K; k += 2 * w) t[k] = pk_max(t[k], t[k + w]);
+    return t[0];
 }
 
 struct PassResult {
@@ -132,11 +149,14 @@ template <int K, bool REV>
 __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& t, int32_t target,
                                uint32_t* bnd_g)
 {
-    constexpr int W = kVirtPerPair * K;
+    constexpr int KS = kstride(K);
+    constexpr int W = kVirtPerPair * K;   // columns of one strip
+    constexpr int WA = kVirtPerPair * KS; // profile columns reserved per row
     const int lane = threadIdx.x;
     const int g = lane >> 4, l = lane & 15;
     const int alpha = a.alpha;
-    const int rowstride = W * 2;
+    const int rowstride = WA * 2;
+    const uint32_t lane0_mask = l == 0 ? 0xFFFFFFFFu : 0u;
     uint8_t* gb = sm.group(g);
     int16_t* prof = (int16_t*)gb;
     uint16_t* ring_code = (uint16_t*)(gb + sm.L.ring_code_off);
@@ -158,6 +178,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         steps_max = max(steps_max, info[q * 8 + 1]);
     }
 
+    const bool multi_strip = strips_max > 1;
     const uint32_t OPEN2 = pack16(-a.gap_open, -a.gap_open);
     const uint32_t EXT2 = pack16(-a.gap_extend, -a.gap_extend);
     const int v_lo = l, v_hi = l + 16;
@@ -171,15 +192,16 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
 
         // ---- profile of this strip: prof[a*W + j] = s(code a, column strip_base+j)
         __syncthreads();
-        for (int j = l; j < W; j += 16) {
-            const int c = strip_base + j;
+        for (int j = l; j < WA; j += 16) {
+            const int vj = j / KS, kj = j - vj * KS;
+            const int c = strip_base + vj * K + kj;
             int code = -1;
-            if (live && c < t.nc) {
-                code = lut[t.cols[t.rev ? t.nc - 1 - c : c]];
+            if (live && kj < K && c < t.nc) {
+                code = lut[t.cols[t.rev ? t.nc_full - 1 - c : c]];
                 if (code >= alpha) code = -1;
             }
             for (int r = 0; r <= alpha; ++r)
-                prof[r * W + j] = (code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg;
+                prof[r * WA + j] = (code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg;
         }
         // ---- row ring: rows [-32, 0) padding, [0, 32) now, [32, 64) in flight
         {
@@ -191,8 +213,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         RowFetch pf = row_fetch(t, 1, l, bnd_g, use_bnd);
         __syncthreads();
 
-        const uint8_t* prof_lo = (const uint8_t*)prof + v_lo * K * 2;
-        const uint8_t* prof_hi = (const uint8_t*)prof + v_hi * K * 2;
+        const uint8_t* prof_lo = (const uint8_t*)prof + v_lo * KS * 2;
+        const uint8_t* prof_hi = (const uint8_t*)prof + v_hi * KS * 2;
 
         uint32_t Hcol[K], Fcol[K];
         uint32_t snap[REV ? 1 : K];
@@ -218,16 +240,14 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 // left inputs: neighbour's outputs of the previous step; virtual lane 0
                 // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
                 uint32_t xH = row_ror1(Hout), xE = row_ror1(Eout);
-                const uint32_t bnd = ring_bnd[s & (kRing - 1)];
-                if (l == 0) {
-                    xH = __builtin_amdgcn_perm(xH, bnd, 0x05040100u);
-                    xE = __builtin_amdgcn_perm(xE, bnd, 0x05040302u);
-                }
+                const uint32_t bnd = multi_strip ? ring_bnd[s & (kRing - 1)] : 0u;
+                xH = bfi(lane0_mask, __builtin_amdgcn_perm(xH, bnd, 0x05040100u), xH);
+                xE = bfi(lane0_mask, __builtin_amdgcn_perm(xE, bnd, 0x05040302u), xE);
                 const uint32_t off_lo = ring_code[(s - v_lo) & (kRing - 1)];
                 const uint32_t off_hi = ring_code[(s - v_hi) & (kRing - 1)];
-                uint32_t plo[K / 2], phi[K / 2];
-                prof_load<K>(prof_lo + off_lo, plo);
-                prof_load<K>(prof_hi + off_hi, phi);
+                uint32_t plo[KS / 2], phi[KS / 2];
+                prof_load<KS>(prof_lo + off_lo, plo);
+                prof_load<KS>(prof_hi + off_hi, phi);
 
                 uint32_t E = xE, Hd = Hdiag0;
 #pragma unroll
@@ -253,9 +273,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                         bnd_g[row] = __builtin_amdgcn_perm(Eout, Hout, 0x07060302u);
                 }
 
-                uint32_t rm = Hcol[0];
-#pragma unroll
-                for (int k = 1; k < K; ++k) rm = pk_max(rm, Hcol[k]);
+                const uint32_t rm = tree_max<K>(Hcol);
                 if constexpr (!REV) {
                     const uint32_t m = pk_lt_mask(best, rm);
                     best = pk_max(best, rm);
@@ -351,17 +369,28 @@ struct FwdOut {
     int32_t score, ref_end, query_end;
 };
 
-__device__ __forceinline__ Task forward_task(const KernelArgs& a, int64_t p)
+__device__ __forceinline__ Task empty_task(const KernelArgs& a, int rev)
 {
     Task t;
+    t.rows = a.ref; t.cols = a.qry;
+    t.nr = t.nc = t.nr_full = t.nc_full = 0;
+    t.rev = rev;
+    return t;
+}
+
+__device__ __forceinline__ Task forward_task(const KernelArgs& a, int64_t p)
+{
+    if (p < 0) return empty_task(a, 0);
+    Task t;
     t.rev = 0;
-    if (p < 0) { t.rows = a.ref; t.cols = a.qry; t.nr = 0; t.nc = 0; return t; }
     const uint8_t* A = a.ref + a.ref_off[p];
     const uint8_t* B = a.qry + a.qry_off[p];
     const int32_t la = (int32_t)(a.ref_off[p + 1] - a.ref_off[p]);
     const int32_t lb = (int32_t)(a.qry_off[p + 1] - a.qry_off[p]);
     if (la < lb) { t.rows = B; t.nr = lb; t.cols = A; t.nc = la; } // kernel.cpp:248-253
     else { t.rows = A; t.nr = la; t.cols = B; t.nc = lb; }
+    t.nr_full = t.nr;
+    t.nc_full = t.nc;
     return t;
 }
 
@@ -381,15 +410,62 @@ __device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p,
     return o;
 }
 
-__device__ __forceinline__ Task reverse_task(const KernelArgs& a, int64_t p, const FwdOut& f)
+// Sum over the forward column prefix of each residue's best possible score: an
+// upper bound of the substitution part of any alignment inside the prefix.
+__device__ __forceinline__ int32_t column_bound(const KernelArgs& a, const Smem& sm, int64_t p, const FwdOut& f)
 {
+    const int l = threadIdx.x & 15;
+    int32_t u = 0;
+    if (p >= 0 && f.score > 0) {
+        const int32_t la = (int32_t)(a.ref_off[p + 1] - a.ref_off[p]);
+        const int32_t lb = (int32_t)(a.qry_off[p + 1] - a.qry_off[p]);
+        const bool ref_rows = !(la < lb);
+        const uint8_t* C = ref_rows ? a.qry + a.qry_off[p] : a.ref + a.ref_off[p];
+        const int32_t ec = ref_rows ? f.query_end : f.ref_end;
+        const uint8_t* lut = sm.lut();
+        const int16_t* cm = sm.colmax();
+        for (int32_t j = l; j < ec; j += 16) {
+            const int code = lut[C[j]];
+            u += code < a.alpha ? (int32_t)cm[code] : 0;
+        }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) u += __shfl_xor(u, o, 16);
+    return u;
+}
+
+// Reverse pass on the prefixes A[0:ref_end) x B[0:query_end), read backwards;
+// orientation by the two end coordinates as in kernel.cpp:444-459.
+//
+// Exact extent bound: the forward winner has the smallest row R* among all cells
+// scoring S, so every score-S alignment inside the prefixes ends on R*, i.e. at
+// index 0 of the reversed forward-row sequence.  Such an alignment uses at most
+// ec columns on the diagonal and its vertical gaps cost at least gap_min per
+// position, so it spans at most ec + (U - S) / gap_min forward-row positions,
+// with U = column_bound.  Cells farther along that axis cannot reach S, and
+// cells nearer do not depend on them, so the pass stops there.
+__device__ __forceinline__ Task reverse_task(const KernelArgs& a, int64_t p, const FwdOut& f, int32_t ubound)
+{
+    if (p < 0 || f.score <= 0) return empty_task(a, 1);
     Task t;
     t.rev = 1;
-    if (p < 0 || f.score <= 0) { t.rows = a.ref; t.cols = a.qry; t.nr = 0; t.nc = 0; return t; }
     const uint8_t* A = a.ref + a.ref_off[p];
     const uint8_t* B = a.qry + a.qry_off[p];
-    if (f.ref_end < f.query_end) { t.rows = B; t.nr = f.query_end; t.cols = A; t.nc = f.ref_end; } // 444-451
-    else { t.rows = A; t.nr = f.ref_end; t.cols = B; t.nc = f.query_end; }                    // 452-459
+    const bool rev_rows_a = !(f.ref_end < f.query_end);
+    if (!rev_rows_a) { t.rows = B; t.nr = f.query_end; t.cols = A; t.nc = f.ref_end; } // 444-451
+    else { t.rows = A; t.nr = f.ref_end; t.cols = B; t.nc = f.query_end; }            // 452-459
+    t.nr_full = t.nr;
+    t.nc_full = t.nc;
+    if (a.gap_min > 0) {
+        const int32_t la = (int32_t)(a.ref_off[p + 1] - a.ref_off[p]);
+        const int32_t lb = (int32_t)(a.qry_off[p + 1] - a.qry_off[p]);
+        const bool fwd_rows_a = !(la < lb);
+        const int32_t ec = fwd_rows_a ? f.query_end : f.ref_end;
+        const int64_t span = (int64_t)ec + (int64_t)max(0, ubound - f.score) / a.gap_min;
+        const int32_t lim = (int32_t)min<int64_t>(span, 0x7FFFFFFF);
+        if (rev_rows_a == fwd_rows_a) t.nr = min(t.nr, lim);
+        else t.nc = min(t.nc, lim);
+    }
     return t;
 }
 
@@ -402,8 +478,8 @@ __device__ __forceinline__ void reverse_result(const KernelArgs& a, int64_t p, c
     if (key == 0) { atomicOr(a.status + kStatusRevMiss, 1); return; }
     const int32_t col = (int32_t)(0xFFFFFu - (uint32_t)((key >> 20) & 0xFFFFFu));
     const int32_t row = (int32_t)(0xFFFFFu - (uint32_t)(key & 0xFFFFFu));
-    const int32_t orow = t.nr - 1 - row; // kernel.cpp:579
-    const int32_t ocol = t.nc - 1 - col; // kernel.cpp:580
+    const int32_t orow = t.nr_full - 1 - row; // kernel.cpp:579
+    const int32_t ocol = t.nc_full - 1 - col; // kernel.cpp:580
     if (f.ref_end < f.query_end) { *query_begin = orow; *ref_begin = ocol; } // 591-595
     else { *ref_begin = orow; *query_begin = ocol; }                       // 596-600
 }
@@ -412,6 +488,7 @@ __device__ __forceinline__ void stage_tables(const KernelArgs& a, const Smem& sm
 {
     for (int i = threadIdx.x; i < 256; i += kWave) sm.lut()[i] = a.lut[i];
     for (int i = threadIdx.x; i < a.alpha * a.alpha; i += kWave) sm.sub()[i] = a.sub[i];
+    for (int i = threadIdx.x; i < a.alpha; i += kWave) sm.colmax()[i] = a.colmax[i];
     __syncthreads();
 }
 
@@ -429,7 +506,7 @@ template <int K>
 __global__ __launch_bounds__(64) void k_plain_forward(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    Smem sm{smem_raw, lds_layout(kVirtPerPair * K, a.alpha)};
+    Smem sm{smem_raw, lds_layout(kVirtPerPair * kstride(K), a.alpha)};
     stage_tables(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
@@ -453,7 +530,7 @@ template <int K>
 __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    Smem sm{smem_raw, lds_layout(kVirtPerPair * K, a.alpha)};
+    Smem sm{smem_raw, lds_layout(kVirtPerPair * kstride(K), a.alpha)};
     stage_tables(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
@@ -466,7 +543,7 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
             f.ref_end = a.out[kOutRefEnd * a.n + p];
             f.query_end = a.out[kOutQueryEnd * a.n + p];
         }
-        const Task t = reverse_task(a, p, f);
+        const Task t = reverse_task(a, p, f, column_bound(a, sm, p, f));
         const PassResult r = run_pass<K, true>(a, sm, t, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, t, r.key, &rb, &qb);
@@ -483,7 +560,7 @@ template <int K>
 __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    Smem sm{smem_raw, lds_layout(kVirtPerPair * K, a.alpha)};
+    Smem sm{smem_raw, lds_layout(kVirtPerPair * kstride(K), a.alpha)};
     stage_tables(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
@@ -497,7 +574,7 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
         const Task tf = forward_task(a, p);
         const PassResult rf = run_pass<K, false>(a, sm, tf, 0, bnd_g);
         const FwdOut f = forward_result(a, p, rf.key, a.status);
-        const Task tr = reverse_task(a, p, f);
+        const Task tr = reverse_task(a, p, f, column_bound(a, sm, p, f));
         const PassResult rr = run_pass<K, true>(a, sm, tr, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, tr, rr.key, &rb, &qb);
@@ -529,19 +606,30 @@ static const void* kernel_ptr(int kind)
     }
 }
 
+#define GBSW_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
+
 static const void* kernel_for(int kind, int K)
 {
     switch (K) {
-    case 2: return kernel_ptr<2>(kind);
-    case 4: return kernel_ptr<4>(kind);
-    case 8: return kernel_ptr<8>(kind);
-    case 16: return kernel_ptr<16>(kind);
-    case 32: return kernel_ptr<32>(kind);
+#define GBSW_CASE(k) case k: return kernel_ptr<k>(kind);
+        GBSW_K_LIST(GBSW_CASE)
+#undef GBSW_CASE
     default: return nullptr;
     }
 }
 
-bool kernel_supported(int K) { return K == 2 || K == 4 || K == 8 || K == 16 || K == 32; }
+bool kernel_supported(int K) { return kernel_for(kPlainForward, K) != nullptr; }
+
+const int* kernel_k_list(int* count)
+{
+    static const int ks[] = {
+#define GBSW_ITEM(k) k,
+        GBSW_K_LIST(GBSW_ITEM)
+#undef GBSW_ITEM
+    };
+    *count = (int)(sizeof(ks) / sizeof(ks[0]));
+    return ks;
+}
 
 // Workgroups of one kernel that fit on a CU at this LDS size (0 on error).
 int kernel_occupancy(int kind, int K, int lds_bytes)

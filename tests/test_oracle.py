@@ -350,3 +350,68 @@ def test_orientation_tie_fixtures():
     for ref, qry, exp in fx["cases"]:
         assert ol.align(ref, qry, sc) == tuple(exp)
         assert kernel_model.align(ref, qry, dna_fn(m, mm), go, ge) == tuple(exp)
+
+
+def _reverse_winner(R, C, s, go, ge, S, nr_lim=None, nc_lim=None):
+    """Reverse-pass winner over reversed R (rows) and C (columns), optionally
+    computing only the first nr_lim rows / nc_lim columns: the smallest column t
+    holding a cell equal to S, then its first row."""
+    nr, nc = len(R), len(C)
+    nr_c = nr if nr_lim is None else min(nr, nr_lim)
+    nc_c = nc if nc_lim is None else min(nc, nc_lim)
+    NEG = -(10 ** 9)
+    Hp = [0] * (nc_c + 1)
+    Fp = [NEG] * (nc_c + 1)
+    best = None
+    for i in range(1, nr_c + 1):
+        rc = R[nr - i]
+        H = [0] * (nc_c + 1)
+        E = NEG
+        for j in range(1, nc_c + 1):
+            E = max(E + ge, H[j - 1] + go)
+            F = max(Fp[j] + ge, Hp[j] + go)
+            Fp[j] = F
+            H[j] = max(0, Hp[j - 1] + s(rc, C[nc - j]), E, F)
+            if H[j] == S and (best is None or j - 1 < best[0]):
+                best = (j - 1, i)
+        Hp = H
+    return best
+
+
+def test_reverse_extent_bound_is_exact():
+    """The engine stops the reverse pass after ec + (U - S) // g positions along
+    the forward-row axis (U: sum of the best score of each forward-column prefix
+    residue, g: cheapest gap position).  Check on tie-heavy and random pairs that
+    the winner of the truncated reverse DP equals the full one."""
+    cases = _tie_heavy_cases(41, 150)
+    rng = random.Random(42)
+    for _ in range(150):
+        a = rand_seq(rng, rng.randint(1, 60), "ACG")
+        cases.append((a, a[rng.randint(0, len(a) - 1):] + rand_seq(rng, rng.randint(0, 20), "ACGT")))
+    for scheme in [(1, -3, -3, -1), (3, -3, -6, -1), (2, -1, -2, -2)]:
+        m, mm, go, ge = scheme
+        sc = ol.dna_scoring(m, mm, go, ge)
+        fn = dna_fn(m, mm)
+        checked = 0
+        for ref, qry in cases:
+            S, rb, ea, qb, eb = ol.align(ref, qry, sc)
+            if S == 0:
+                continue
+            fwd_rows_ref = not (len(ref) < len(qry))
+            rev_rows_ref = not (ea < eb)
+            ec = eb if fwd_rows_ref else ea
+            U = max(m, 0) * ec
+            g = min(-go, -ge)
+            lim = ec + (U - S) // g
+            Rr, Cr = (ref[:ea], qry[:eb]) if rev_rows_ref else (qry[:eb], ref[:ea])
+            full = _reverse_winner(Rr, Cr, fn, go, ge, S)
+            if rev_rows_ref == fwd_rows_ref:
+                cut = _reverse_winner(Rr, Cr, fn, go, ge, S, nr_lim=lim)
+            else:
+                cut = _reverse_winner(Rr, Cr, fn, go, ge, S, nc_lim=lim)
+            assert cut == full, (ref, qry, scheme, lim)
+            t, i = full
+            orow, ocol = len(Rr) - i, len(Cr) - 1 - t
+            assert (rb, qb) == ((orow, ocol) if rev_rows_ref else (ocol, orow))
+            checked += 1
+        assert checked > 200
