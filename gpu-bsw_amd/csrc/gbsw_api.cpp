@@ -73,10 +73,14 @@ struct Bucket {
 
 } // namespace
 
+constexpr int kAuxStreams = 3;
+
 struct gbsw_context {
     int device = 0;
     int num_cu = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux[kAuxStreams] = {nullptr, nullptr, nullptr}; // concurrent bucket launches
+    hipEvent_t ev_fork = nullptr, ev_join[kAuxStreams] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> kev; // per-launch event pairs
     DeviceBuffer bnd, queue, status, sub, lut, colmax;
@@ -97,6 +101,12 @@ struct gbsw_batch {
     std::vector<Bucket> buckets;
     int buckets_for_maxk = -1;
     gbsw_stats stats{};
+    struct Launch {
+        int kind, K, grid;
+        int64_t pairs, cells;
+        double ms;
+    };
+    std::vector<Launch> launches; // of the last gbsw_batch_align
 };
 
 namespace {
@@ -234,20 +244,20 @@ KernelArgs base_args(gbsw_batch* b, const gbsw_scoring* sc, int max_sub)
     return a;
 }
 
-int launch_timed(gbsw_context* ctx, int kind, int K, int grid, int lds, KernelArgs& a, size_t ev_slot)
+// Launch one kernel between two timing events.  `bnd` must hold
+// grid * kPairsPerWave * bnd_rows words for this launch alone.
+int launch_timed(gbsw_context* ctx, hipStream_t stream, int kind, int K, int grid, int lds, KernelArgs& a,
+                 uint32_t* bnd, size_t ev_slot)
 {
-    a.bnd = ctx->bnd.as<uint32_t>();
-    const size_t need = (size_t)grid * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t);
-    HIP_TRY(ctx->bnd.reserve(need));
-    a.bnd = ctx->bnd.as<uint32_t>();
+    a.bnd = bnd;
     while (ctx->kev.size() < 2 * (ev_slot + 1)) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
         ctx->kev.push_back(e);
     }
-    HIP_TRY(hipEventRecord(ctx->kev[2 * ev_slot], ctx->stream));
-    HIP_TRY(kernel_launch(kind, K, grid, lds, ctx->stream, a));
-    HIP_TRY(hipEventRecord(ctx->kev[2 * ev_slot + 1], ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->kev[2 * ev_slot], stream));
+    HIP_TRY(kernel_launch(kind, K, grid, lds, stream, a));
+    HIP_TRY(hipEventRecord(ctx->kev[2 * ev_slot + 1], stream));
     return GBSW_OK;
 }
 
@@ -283,6 +293,11 @@ int gbsw_context_create(int32_t device, gbsw_context** out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev_start);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stop);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+    for (int i = 0; i < kAuxStreams && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = ctx->status.reserve(sizeof(int32_t) * kStatusWords);
     if (e == hipSuccess) e = ctx->queue.reserve(sizeof(uint32_t) * 64);
     if (e == hipSuccess) e = ctx->sub.reserve(sizeof(int16_t) * 1024);
@@ -305,6 +320,11 @@ int gbsw_context_destroy(gbsw_context* ctx)
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (int i = 0; i < kAuxStreams; ++i) {
+        if (ctx->aux[i]) { (void)hipStreamSynchronize(ctx->aux[i]); (void)hipStreamDestroy(ctx->aux[i]); }
+        if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
+    }
     ctx->bnd.release(); ctx->queue.release(); ctx->status.release(); ctx->sub.release(); ctx->lut.release();
     ctx->colmax.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -443,6 +463,7 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
     HIP_TRY(zero_u32((uint32_t*)ctx->status.ptr, kStatusWords, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
     size_t slot = 0;
+    b->launches.clear();
     if (mode == GBSW_PLAIN) {
         // one geometry for the whole batch, sized by its longest shorter sequence
         const int K = k_for_cols(b->max_cols, kmax);
@@ -451,11 +472,14 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
         a.order = nullptr;
         a.order_n = b->n;
         const int gf = grid_for(ctx, kPlainForward, K, lds, nquads);
-        rc = launch_timed(ctx, kPlainForward, K, gf, lds, a, slot++);
-        if (rc) return rc;
         const int gr = grid_for(ctx, kPlainReverse, K, lds, nquads);
-        rc = launch_timed(ctx, kPlainReverse, K, gr, lds, a, slot++);
+        HIP_TRY(ctx->bnd.reserve((size_t)std::max(gf, gr) * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t)));
+        rc = launch_timed(ctx, ctx->stream, kPlainForward, K, gf, lds, a, ctx->bnd.as<uint32_t>(), slot++);
         if (rc) return rc;
+        b->launches.push_back({kPlainForward, K, gf, b->n, b->cells, 0.0});
+        rc = launch_timed(ctx, ctx->stream, kPlainReverse, K, gr, lds, a, ctx->bnd.as<uint32_t>(), slot++);
+        if (rc) return rc;
+        b->launches.push_back({kPlainReverse, K, gr, b->n, 0, 0.0});
         b->stats.buckets = 1;
     } else {
         rc = build_buckets(b);
@@ -463,16 +487,39 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
         if (ctx->queue.bytes < sizeof(uint32_t) * b->buckets.size())
             HIP_TRY(ctx->queue.reserve(sizeof(uint32_t) * b->buckets.size()));
         HIP_TRY(zero_u32((uint32_t*)ctx->queue.ptr, (int)b->buckets.size(), ctx->stream));
+        // Buckets run concurrently on 1 + kAuxStreams streams, the ones whose single
+        // work item takes longest (largest K) first, so a bucket with few long
+        // pairs overlaps the bulk instead of running alone at the end.
+        std::vector<size_t> order(b->buckets.size());
+        std::vector<int> grids(b->buckets.size()), ldss(b->buckets.size());
+        size_t bnd_words = 0;
         for (size_t i = 0; i < b->buckets.size(); ++i) {
+            order[i] = i;
             const Bucket& bk = b->buckets[i];
-            const int lds = lds_for(bk.K, b->alpha);
-            const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
+            ldss[i] = lds_for(bk.K, b->alpha);
+            grids[i] = grid_for(ctx, kBatched, bk.K, ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
+            bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+        }
+        std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return b->buckets[x].K > b->buckets[y].K; });
+        HIP_TRY(ctx->bnd.reserve(bnd_words * sizeof(uint32_t)));
+        HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+        for (int i = 0; i < kAuxStreams; ++i) HIP_TRY(hipStreamWaitEvent(ctx->aux[i], ctx->ev_fork, 0));
+        size_t bnd_at = 0;
+        for (size_t j = 0; j < order.size(); ++j) {
+            const size_t i = order[j];
+            const Bucket& bk = b->buckets[i];
             a.order = b->order.as<int32_t>() + bk.begin;
             a.order_n = bk.count;
             a.queue = (uint32_t*)ctx->queue.ptr + i;
-            const int grid = grid_for(ctx, kBatched, bk.K, lds, nquads);
-            rc = launch_timed(ctx, kBatched, bk.K, grid, lds, a, slot++);
+            hipStream_t st = (j % (kAuxStreams + 1)) == 0 ? ctx->stream : ctx->aux[j % (kAuxStreams + 1) - 1];
+            rc = launch_timed(ctx, st, kBatched, bk.K, grids[i], ldss[i], a, ctx->bnd.as<uint32_t>() + bnd_at, slot++);
             if (rc) return rc;
+            bnd_at += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+            b->launches.push_back({kBatched, bk.K, grids[i], bk.count, bk.cells, 0.0});
+        }
+        for (int i = 0; i < kAuxStreams; ++i) {
+            HIP_TRY(hipEventRecord(ctx->ev_join[i], ctx->aux[i]));
+            HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join[i], 0));
         }
         b->stats.buckets = (int32_t)b->buckets.size();
     }
@@ -484,13 +531,14 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     b->stats.device_ms = ms;
     b->stats.launches = (int32_t)slot;
-    double kms = 0.0;
+    double kmax_ms = 0.0;
     for (size_t i = 0; i < slot; ++i) {
         float t = 0.f;
         HIP_TRY(hipEventElapsedTime(&t, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
-        kms += t;
+        b->launches[i].ms = t;
+        kmax_ms = std::max(kmax_ms, (double)t);
     }
-    b->stats.main_kernel_ms = kms;
+    b->stats.main_kernel_ms = kmax_ms;
     if (status[kStatusSaturated]) return fail(GBSW_E_RANGE, "a score reached the int16 saturation limit");
     if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
     return GBSW_OK;
@@ -520,6 +568,20 @@ int gbsw_batch_stats(const gbsw_batch* b, gbsw_stats* out)
 {
     if (!b || !out) return fail(GBSW_E_ARGUMENT, "null argument");
     *out = b->stats;
+    return GBSW_OK;
+}
+
+int gbsw_batch_launch_stats(const gbsw_batch* b, int32_t index, gbsw_launch_stat* out)
+{
+    if (!b || !out) return fail(GBSW_E_ARGUMENT, "null argument");
+    if (index < 0 || (size_t)index >= b->launches.size()) return fail(GBSW_E_ARGUMENT, "launch index out of range");
+    const gbsw_batch::Launch& L = b->launches[(size_t)index];
+    out->kind = L.kind;
+    out->columns_per_lane = L.K;
+    out->workgroups = L.grid;
+    out->pairs = L.pairs;
+    out->cells = L.cells;
+    out->ms = L.ms;
     return GBSW_OK;
 }
 

@@ -51,9 +51,15 @@ struct LdsLayout {
 
 __host__ __device__ inline int32_t align_up(int32_t x, int32_t a) { return (x + a - 1) / a * a; }
 
-// Profile columns reserved per virtual lane for K columns: a whole number of
-// LDS read units (b32, b64 or b128 multiples), so every read is aligned.
-__host__ __device__ constexpr int32_t kstride(int32_t K) { return K <= 2 ? 2 : K <= 4 ? 4 : (K + 7) / 8 * 8; }
+// Profile layout of one row for K columns per virtual lane: K/8 blocks of 8
+// columns for all 32 virtual lanes (one b128 read each), then the K%8 remaining
+// columns for all virtual lanes padded to a b32/b64/b128 read (kremstride).
+__host__ __device__ constexpr int32_t kremstride(int32_t K)
+{
+    return K % 8 == 0 ? 0 : K % 8 <= 2 ? 2 : K % 8 <= 4 ? 4 : 8;
+}
+// Profile columns reserved per virtual lane.
+__host__ __device__ constexpr int32_t kstride(int32_t K) { return K / 8 * 8 + kremstride(K); }
 
 __host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha)
 {

@@ -101,23 +101,44 @@ __device__ __forceinline__ void row_store(const RowFetch& f, int chunk, int l, u
     ring_bnd[x1 & (kRing - 1)] = f.d1;
 }
 
-// Load the KS = kstride(K) profile scores reserved for one virtual lane
-// (KS/2 dwords; only the first K are used).
-template <int KS>
-__device__ __forceinline__ void prof_load(const uint8_t* p, uint32_t (&d)[KS / 2])
+// Load the kstride(K) profile scores of one virtual lane v from profile row
+// `row`: K/8 b128 blocks (block q at q*512 + v*16 bytes) and the remainder at
+// (K/8)*512 + v*kremstride(K)*2.  Dword d holds columns 2d and 2d+1.
+template <int K>
+__device__ __forceinline__ void prof_load(const uint8_t* row, int v, uint32_t (&d)[kstride(K) / 2])
 {
-    if constexpr (KS % 8 == 0) {
+    constexpr int Q = K / 8, RS = kremstride(K);
 #pragma unroll
-        for (int q = 0; q < KS / 8; ++q) {
-            const uint4 v = *(const uint4*)(p + 16 * q);
-            d[4 * q + 0] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
-        }
-    } else if constexpr (KS == 4) {
-        const uint2 v = *(const uint2*)p;
-        d[0] = v.x; d[1] = v.y;
+    for (int q = 0; q < Q; ++q) {
+        const uint4 x = *(const uint4*)(row + q * 512 + v * 16);
+        d[4 * q + 0] = x.x; d[4 * q + 1] = x.y; d[4 * q + 2] = x.z; d[4 * q + 3] = x.w;
+    }
+    const uint8_t* r = row + Q * 512 + v * RS * 2;
+    if constexpr (RS == 8) {
+        const uint4 x = *(const uint4*)r;
+        d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y; d[4 * Q + 2] = x.z; d[4 * Q + 3] = x.w;
+    } else if constexpr (RS == 4) {
+        const uint2 x = *(const uint2*)r;
+        d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y;
+    } else if constexpr (RS == 2) {
+        d[4 * Q] = *(const uint32_t*)r;
+    }
+}
+
+// Profile column index j (0..32*kstride(K)) -> (virtual lane, column within it);
+// k >= K marks padding.
+template <int K>
+__device__ __forceinline__ void prof_slot(int j, int& v, int& k)
+{
+    constexpr int Q = K / 8, RS = kremstride(K);
+    if (j < Q * 256) {
+        const int q = j >> 8;
+        v = (j & 255) >> 3;
+        k = q * 8 + (j & 7);
     } else {
-        static_assert(KS == 2, "profile stride must be 2, 4 or a multiple of 8");
-        d[0] = *(const uint32_t*)p;
+        const int jj = j - Q * 256;
+        v = RS ? jj / RS : 0;
+        k = Q * 8 + (RS ? jj - v * RS : 0);
     }
 }
 
@@ -193,7 +214,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         // ---- profile of this strip: prof[a*W + j] = s(code a, column strip_base+j)
         __syncthreads();
         for (int j = l; j < WA; j += 16) {
-            const int vj = j / KS, kj = j - vj * KS;
+            int vj, kj;
+            prof_slot<K>(j, vj, kj);
             const int c = strip_base + vj * K + kj;
             int code = -1;
             if (live && kj < K && c < t.nc) {
@@ -213,8 +235,6 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         RowFetch pf = row_fetch(t, 1, l, bnd_g, use_bnd);
         __syncthreads();
 
-        const uint8_t* prof_lo = (const uint8_t*)prof + v_lo * KS * 2;
-        const uint8_t* prof_hi = (const uint8_t*)prof + v_hi * KS * 2;
 
         uint32_t Hcol[K], Fcol[K];
         uint32_t snap[REV ? 1 : K];
@@ -235,8 +255,11 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 pf = row_fetch(t, ck + 1, l, bnd_g, use_bnd);
             }
             __syncthreads();
+            // two steps per iteration (manual unroll: the DPP moves are convergent,
+            // so the compiler will not unroll a runtime trip count); a step past
+            // steps_max only meets padding rows
             const int s_end = min(s0 + kChunk, steps_max);
-            for (int s = s0; s < s_end; ++s) {
+            auto step = [&](const int s) {
                 // left inputs: neighbour's outputs of the previous step; virtual lane 0
                 // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
                 uint32_t xH = row_ror1(Hout), xE = row_ror1(Eout);
@@ -246,8 +269,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 const uint32_t off_lo = ring_code[(s - v_lo) & (kRing - 1)];
                 const uint32_t off_hi = ring_code[(s - v_hi) & (kRing - 1)];
                 uint32_t plo[KS / 2], phi[KS / 2];
-                prof_load<KS>(prof_lo + off_lo, plo);
-                prof_load<KS>(prof_hi + off_hi, phi);
+                prof_load<K>((const uint8_t*)prof + off_lo, v_lo, plo);
+                prof_load<K>((const uint8_t*)prof + off_hi, v_hi, phi);
 
                 uint32_t E = xE, Hd = Hdiag0;
 #pragma unroll
@@ -267,9 +290,9 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 Hdiag0 = xH;
 
                 // the last virtual lane hands its column to the next strip
-                if (write_bnd && l == 15) {
+                if (multi_strip) {
                     const int row = s - v_hi;
-                    if (row >= 0 && row < t.nr)
+                    if (write_bnd && l == 15 && row >= 0 && row < t.nr)
                         bnd_g[row] = __builtin_amdgcn_perm(Eout, Hout, 0x07060302u);
                 }
 
@@ -301,6 +324,10 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                     }
                 }
                 rowpk = raw(as_u(rowpk) + (u16x2){1, 1});
+            };
+            for (int s = s0; s < s_end; s += 2) {
+                step(s);
+                step(s + 1);
             }
         }
 

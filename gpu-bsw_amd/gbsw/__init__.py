@@ -54,11 +54,22 @@ class Stats(ctypes.Structure):
     ]
 
 
+class LaunchStat(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("columns_per_lane", ctypes.c_int32),
+        ("workgroups", ctypes.c_int32),
+        ("pairs", ctypes.c_int64),
+        ("cells", ctypes.c_int64),
+        ("ms", ctypes.c_double),
+    ]
+
+
 # Every symbol include/gbsw.h declares (checked by the CPU test suite).
 EXPORTED = (
     "gbsw_context_create", "gbsw_context_destroy", "gbsw_last_error", "gbsw_batch_create",
     "gbsw_batch_destroy", "gbsw_batch_align", "gbsw_batch_results", "gbsw_batch_device_results",
-    "gbsw_batch_stats", "gbsw_align", "gbsw_device_memory", "gbsw_version",
+    "gbsw_batch_stats", "gbsw_batch_launch_stats", "gbsw_align", "gbsw_device_memory", "gbsw_version",
 )
 
 _lib = None
@@ -88,6 +99,7 @@ def load_library():
     lib.gbsw_batch_results.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.gbsw_batch_device_results.argtypes = [vp, ctypes.POINTER(vp)]
     lib.gbsw_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    lib.gbsw_batch_launch_stats.argtypes = [vp, i32, ctypes.POINTER(LaunchStat)]
     lib.gbsw_align.argtypes = [vp, ctypes.POINTER(Scoring), i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.gbsw_device_memory.argtypes = [i32, ctypes.POINTER(ctypes.c_uint64)]
     for name in EXPORTED:
@@ -199,6 +211,15 @@ class Batch:
         st = Stats()
         _check(self._lib.gbsw_batch_stats(self._h, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def launches(self):
+        """Per-launch statistics of the last align: kind, K, workgroups, pairs, cells, ms."""
+        out = []
+        for i in range(self.stats()["launches"]):
+            ls = LaunchStat()
+            _check(self._lib.gbsw_batch_launch_stats(self._h, i, ctypes.byref(ls)))
+            out.append({k: getattr(ls, k) for k, _ in LaunchStat._fields_})
+        return out
 
     def close(self):
         if self._h:

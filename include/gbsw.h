@@ -62,7 +62,7 @@ typedef struct {
 
 typedef struct {
     double device_ms;       /* last gbsw_batch_align: device time, hipEvents        */
-    double main_kernel_ms;  /* time of the dominant kernel launch(es)               */
+    double main_kernel_ms;  /* duration of its longest kernel launch                */
     int64_t cells;          /* sum of |ref|*|query| (the reference's "Total Cells") */
     int32_t launches;       /* kernel launches issued                               */
     int32_t buckets;        /* length buckets used (BATCHED)                        */
@@ -98,6 +98,17 @@ int gbsw_batch_results(gbsw_batch* batch, int32_t* score, int32_t* ref_begin, in
 int gbsw_batch_device_results(gbsw_batch* batch, const int32_t** d_out);
 
 int gbsw_batch_stats(const gbsw_batch* batch, gbsw_stats* out);
+
+/* One kernel launch of the last gbsw_batch_align (index 0 .. stats.launches-1). */
+typedef struct {
+    int32_t kind;             /* 0 plain forward, 1 plain reverse, 2 batched fwd+rev  */
+    int32_t columns_per_lane; /* K: columns per virtual lane (strip = 32*K columns)  */
+    int32_t workgroups;       /* grid size (64-lane workgroups)                       */
+    int64_t pairs;            /* pairs processed by the launch                        */
+    int64_t cells;            /* their sum of |ref|*|query| (0 for the reverse launch)*/
+    double ms;                /* duration, hipEvents around the launch                */
+} gbsw_launch_stat;
+int gbsw_batch_launch_stats(const gbsw_batch* batch, int32_t index, gbsw_launch_stat* out);
 
 /* One call, host memory in and out: upload, align, download (the drop-in path). */
 int gbsw_align(gbsw_context* ctx, const gbsw_scoring* scoring, int32_t mode, int64_t n,
