@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (rocprof FETCH/WRITE calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs are independent; "
@@ -93,6 +95,15 @@ def main():
     w = synth.batched_dna(args.pairs)
     cells = w.cells()
     log("workload %s: %d pairs, %.4g cells, generated in %.1f s" % (w.name, w.n, cells, time.perf_counter() - t))
+
+    if args.calib_copy_mib:
+        import torch
+        src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB device-to-device" % args.calib_copy_mib)
+        del src, dst
 
     ctx = gbsw.Context(0)
     batch = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -74,6 +75,18 @@ struct Bucket {
 } // namespace
 
 constexpr int kAuxStreams = 3;
+
+// Streams used for the length buckets of the batched form: 1 + kAuxStreams by
+// default; GBSW_STREAMS=1 serialises the buckets (per-kernel timing and counters).
+int bucket_streams()
+{
+    static const int n = [] {
+        const char* e = std::getenv("GBSW_STREAMS");
+        const int v = e ? std::atoi(e) : kAuxStreams + 1;
+        return std::max(1, std::min(kAuxStreams + 1, v));
+    }();
+    return n;
+}
 
 struct gbsw_context {
     int device = 0;
@@ -511,7 +524,8 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
             a.order = b->order.as<int32_t>() + bk.begin;
             a.order_n = bk.count;
             a.queue = (uint32_t*)ctx->queue.ptr + i;
-            hipStream_t st = (j % (kAuxStreams + 1)) == 0 ? ctx->stream : ctx->aux[j % (kAuxStreams + 1) - 1];
+            const int ns = bucket_streams();
+            hipStream_t st = (j % ns) == 0 ? ctx->stream : ctx->aux[j % ns - 1];
             rc = launch_timed(ctx, st, kBatched, bk.K, grids[i], ldss[i], a, ctx->bnd.as<uint32_t>() + bnd_at, slot++);
             if (rc) return rc;
             bnd_at += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
