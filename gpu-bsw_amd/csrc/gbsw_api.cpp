@@ -110,9 +110,11 @@ struct gbsw_batch {
     int32_t max_rows = 0;        // longest sequence
     int32_t max_cols = 0;        // longest shorter-sequence of a pair
     int64_t cells = 0;
-    // batched form: buckets over the sorted order array (built lazily)
+    // batched form: length buckets.  Counts come from the lengths at upload;
+    // the work order itself is sorted on the device by every gbsw_batch_align.
     std::vector<Bucket> buckets;
-    int buckets_for_maxk = -1;
+    DeviceBuffer bucket_of_len, keys, keys_sorted, vals, sort_temp;
+    size_t sort_temp_bytes = 0;
     gbsw_stats stats{};
     struct Launch {
         int kind, K, grid;
@@ -165,47 +167,62 @@ int grid_for(gbsw_context* ctx, int kind, int K, int lds, int64_t nquads)
     return (int)std::max<int64_t>(1, std::min<int64_t>(full, nquads));
 }
 
-int build_buckets(gbsw_batch* b)
+// Length buckets of a batch: pairs whose shorter sequence gets the same K.
+// Bucket i covers slots [begin, begin + count) of the device-sorted order.
+int setup_buckets(gbsw_batch* b)
 {
     const int kmax = max_k_for(b->alpha);
-    if (b->buckets_for_maxk == kmax) return GBSW_OK;
-    // bucket by the K that covers the shorter sequence; inside a bucket the
-    // longest pairs go first (longest-processing-time order for the queue)
-    std::vector<int32_t> idx((size_t)b->n);
-    std::vector<int8_t> kb((size_t)b->n);
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    std::vector<uint8_t> bucket_of_len(65536);
+    for (int m = 0; m < 65536; ++m) {
+        const int K = k_for_cols(m, kmax);
+        int idx = 0;
+        while (idx < nk && ks[idx] != K) ++idx;
+        bucket_of_len[(size_t)m] = (uint8_t)idx;
+    }
+    std::vector<int64_t> count((size_t)nk, 0), cells((size_t)nk, 0);
     for (int64_t p = 0; p < b->n; ++p) {
-        idx[(size_t)p] = (int32_t)p;
-        const int m = std::min(b->len_ref[(size_t)p], b->len_qry[(size_t)p]);
-        kb[(size_t)p] = (int8_t)k_for_cols(m, kmax);
+        const int32_t la = b->len_ref[(size_t)p], lb = b->len_qry[(size_t)p];
+        const int bi = bucket_of_len[(size_t)std::min<int32_t>(std::min(la, lb), 65535)];
+        count[(size_t)bi] += 1;
+        cells[(size_t)bi] += (int64_t)la * lb;
     }
-    std::sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
-        if (kb[(size_t)x] != kb[(size_t)y]) return kb[(size_t)x] < kb[(size_t)y];
-        const int64_t cx = (int64_t)b->len_ref[(size_t)x] * b->len_qry[(size_t)x];
-        const int64_t cy = (int64_t)b->len_ref[(size_t)y] * b->len_qry[(size_t)y];
-        const int32_t mx = std::max(b->len_ref[(size_t)x], b->len_qry[(size_t)x]);
-        const int32_t my = std::max(b->len_ref[(size_t)y], b->len_qry[(size_t)y]);
-        if (mx != my) return mx > my;
-        if (cx != cy) return cx > cy;
-        return x < y;
-    });
     b->buckets.clear();
-    for (int64_t s = 0; s < b->n;) {
+    int64_t at = 0;
+    for (int i = 0; i < nk; ++i) {
+        if (count[(size_t)i] == 0) continue;
         Bucket bk;
-        bk.K = kb[(size_t)idx[(size_t)s]];
-        bk.begin = s;
-        int64_t e = s;
-        while (e < b->n && kb[(size_t)idx[(size_t)e]] == bk.K) {
-            bk.cells += (int64_t)b->len_ref[(size_t)idx[(size_t)e]] * b->len_qry[(size_t)idx[(size_t)e]];
-            ++e;
-        }
-        bk.count = e - s;
+        bk.K = ks[i];
+        bk.begin = at;
+        bk.count = count[(size_t)i];
+        bk.cells = cells[(size_t)i];
+        at += bk.count;
         b->buckets.push_back(bk);
-        s = e;
     }
-    HIP_TRY(b->order.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(1, b->n)));
-    if (b->n > 0)
-        HIP_TRY(hipMemcpy(b->order.ptr, idx.data(), sizeof(int32_t) * (size_t)b->n, hipMemcpyHostToDevice));
-    b->buckets_for_maxk = kmax;
+    const size_t n = (size_t)std::max<int64_t>(1, b->n);
+    HIP_TRY(b->bucket_of_len.reserve(65536));
+    HIP_TRY(hipMemcpy(b->bucket_of_len.ptr, bucket_of_len.data(), 65536, hipMemcpyHostToDevice));
+    HIP_TRY(b->keys.reserve(n * 4));
+    HIP_TRY(b->keys_sorted.reserve(n * 4));
+    HIP_TRY(b->vals.reserve(n * 4));
+    HIP_TRY(b->order.reserve(n * 4));
+    size_t tb = 0;
+    HIP_TRY(sort_pairs(nullptr, tb, nullptr, nullptr, nullptr, nullptr, (int)n, 20, nullptr));
+    HIP_TRY(b->sort_temp.reserve(std::max<size_t>(tb, 16)));
+    b->sort_temp_bytes = tb;
+    return GBSW_OK;
+}
+
+// Device work order of the batched form: keys, then a stable radix sort.
+int sort_work(gbsw_batch* b, hipStream_t stream)
+{
+    if (b->n <= 0) return GBSW_OK;
+    HIP_TRY(bin_keys(b->ref_off.as<uint64_t>(), b->qry_off.as<uint64_t>(), b->n, b->bucket_of_len.as<uint8_t>(),
+                     b->keys.as<uint32_t>(), b->vals.as<int32_t>(), stream));
+    size_t tb = b->sort_temp_bytes;
+    HIP_TRY(sort_pairs(b->sort_temp.ptr, tb, b->keys.as<uint32_t>(), b->keys_sorted.as<uint32_t>(), b->vals.as<int32_t>(),
+                       b->order.as<int32_t>(), (int)b->n, 20, stream));
     return GBSW_OK;
 }
 
@@ -433,6 +450,13 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
                     std::string("batch upload: ") + hipGetErrorString(e));
     }
     b->stats.cells = b->cells;
+    const int brc = setup_buckets(b);
+    if (brc) {
+        const std::string err = g_err;
+        delete b;
+        g_err = err;
+        return brc;
+    }
     *out = b;
     return GBSW_OK;
 }
@@ -495,7 +519,7 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
         b->launches.push_back({kPlainReverse, K, gr, b->n, 0, 0.0});
         b->stats.buckets = 1;
     } else {
-        rc = build_buckets(b);
+        rc = sort_work(b, ctx->stream);
         if (rc) return rc;
         if (ctx->queue.bytes < sizeof(uint32_t) * b->buckets.size())
             HIP_TRY(ctx->queue.reserve(sizeof(uint32_t) * b->buckets.size()));
