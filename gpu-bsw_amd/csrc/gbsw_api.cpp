@@ -96,7 +96,7 @@ struct gbsw_context {
     hipEvent_t ev_fork = nullptr, ev_join[kAuxStreams] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> kev; // per-launch event pairs
-    DeviceBuffer bnd, queue, status, sub, lut, colmax;
+    DeviceBuffer bnd, queue, status, sub, lut, colmax, redo;
 };
 
 struct gbsw_batch {
@@ -126,7 +126,11 @@ struct gbsw_batch {
 
 namespace {
 
-int lds_for(int K, int alpha) { return lds_layout(kVirtPerPair * kstride(K), alpha).total; }
+// LDS bytes of a workgroup; bnd_bytes = 4 (packed int16) or 8 (int32 kernel).
+int lds_for(int K, int alpha, int bnd_bytes = 4)
+{
+    return lds_layout(kVirtPerPair * kstride(K), alpha, bnd_bytes).total;
+}
 
 // Largest compiled K whose LDS footprint fits a 64 KiB workgroup.
 int max_k_for(int alpha)
@@ -262,6 +266,7 @@ KernelArgs base_args(gbsw_batch* b, const gbsw_scoring* sc, int max_sub)
     a.n = b->n;
     a.out = b->out.as<int32_t>();
     a.status = ctx->status.as<int32_t>();
+    a.redo = ctx->redo.as<int32_t>();
     a.lut = ctx->lut.as<uint8_t>();
     a.sub = ctx->sub.as<int16_t>();
     a.colmax = ctx->colmax.as<int16_t>();
@@ -277,7 +282,7 @@ KernelArgs base_args(gbsw_batch* b, const gbsw_scoring* sc, int max_sub)
 // Launch one kernel between two timing events.  `bnd` must hold
 // grid * kPairsPerWave * bnd_rows words for this launch alone.
 int launch_timed(gbsw_context* ctx, hipStream_t stream, int kind, int K, int grid, int lds, KernelArgs& a,
-                 uint32_t* bnd, size_t ev_slot)
+                 void* bnd, size_t ev_slot)
 {
     a.bnd = bnd;
     while (ctx->kev.size() < 2 * (ev_slot + 1)) {
@@ -479,9 +484,9 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
     int max_sub = 0;
     int rc = validate_scoring(b, sc, sub, &max_sub);
     if (rc) return rc;
-    // int16 cells are exact while no score can reach the saturation limit
-    if ((int64_t)b->max_cols * max_sub >= 32767 - max_sub)
-        return fail(GBSW_E_RANGE, "possible score exceeds the int16 range of this engine");
+    // result keys hold scores in 23 bits; int16 saturation is handled per pair
+    if ((int64_t)b->max_cols * max_sub >= (1 << 23))
+        return fail(GBSW_E_RANGE, "possible score exceeds 2^23");
     b->stats = gbsw_stats{};
     b->stats.cells = b->cells;
     if (b->n == 0) return GBSW_OK;
@@ -494,6 +499,7 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
             colmax[(size_t)c] = std::max<int16_t>(colmax[(size_t)c], sub[(size_t)(r * b->alpha + c)]);
     HIP_TRY(hipMemcpyAsync(ctx->colmax.ptr, colmax.data(), sizeof(int16_t) * colmax.size(), hipMemcpyHostToDevice,
                            ctx->stream));
+    HIP_TRY(ctx->redo.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(1, b->n)));
     KernelArgs a = base_args(b, sc, max_sub);
     const int kmax = max_k_for(b->alpha);
     std::vector<std::pair<int, int>> timed; // (event slot, is main kernel)
@@ -511,10 +517,10 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
         const int gf = grid_for(ctx, kPlainForward, K, lds, nquads);
         const int gr = grid_for(ctx, kPlainReverse, K, lds, nquads);
         HIP_TRY(ctx->bnd.reserve((size_t)std::max(gf, gr) * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t)));
-        rc = launch_timed(ctx, ctx->stream, kPlainForward, K, gf, lds, a, ctx->bnd.as<uint32_t>(), slot++);
+        rc = launch_timed(ctx, ctx->stream, kPlainForward, K, gf, lds, a, ctx->bnd.ptr, slot++);
         if (rc) return rc;
         b->launches.push_back({kPlainForward, K, gf, b->n, b->cells, 0.0});
-        rc = launch_timed(ctx, ctx->stream, kPlainReverse, K, gr, lds, a, ctx->bnd.as<uint32_t>(), slot++);
+        rc = launch_timed(ctx, ctx->stream, kPlainReverse, K, gr, lds, a, ctx->bnd.ptr, slot++);
         if (rc) return rc;
         b->launches.push_back({kPlainReverse, K, gr, b->n, 0, 0.0});
         b->stats.buckets = 1;
@@ -565,6 +571,30 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     int32_t status[kStatusWords];
     HIP_TRY(hipMemcpy(status, ctx->status.ptr, sizeof(status), hipMemcpyDeviceToHost));
+    if (status[kStatusSaturated] > 0) {
+        // pairs whose int16 score reached the saturation limit: recompute them
+        // (forward + reverse) with int32 cells
+        int nk = 0;
+        const int* wk = wide_k_list(&nk);
+        int K = wk[nk - 1];
+        for (int i = 0; i < nk; ++i)
+            if (kVirtPerPair * wk[i] >= b->max_cols) { K = wk[i]; break; }
+        const int lds = lds_for(K, b->alpha, 8);
+        const int64_t count = status[kStatusSaturated];
+        a.order = ctx->redo.as<int32_t>();
+        a.order_n = count;
+        a.sat_limit = INT_MAX;
+        a.queue = (uint32_t*)ctx->queue.ptr;
+        HIP_TRY(zero_u32(a.queue, 1, ctx->stream));
+        const int grid = grid_for(ctx, kBatchedWide, K, lds, (count + kPairsPerWave - 1) / kPairsPerWave);
+        HIP_TRY(ctx->bnd.reserve((size_t)grid * kPairsPerWave * (size_t)a.bnd_rows * 8));
+        rc = launch_timed(ctx, ctx->stream, kBatchedWide, K, grid, lds, a, ctx->bnd.ptr, slot++);
+        if (rc) return rc;
+        b->launches.push_back({kBatchedWide, K, grid, count, 0, 0.0});
+        HIP_TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(status, ctx->status.ptr, sizeof(status), hipMemcpyDeviceToHost));
+    }
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     b->stats.device_ms = ms;
@@ -577,7 +607,6 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
         kmax_ms = std::max(kmax_ms, (double)t);
     }
     b->stats.main_kernel_ms = kmax_ms;
-    if (status[kStatusSaturated]) return fail(GBSW_E_RANGE, "a score reached the int16 saturation limit");
     if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
     return GBSW_OK;
 }

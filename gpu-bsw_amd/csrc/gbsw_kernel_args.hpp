@@ -29,9 +29,10 @@ struct KernelArgs {
     int64_t order_n;           // slots in this launch
     uint32_t* queue;           // persistent work counter (batched)
     int32_t* out;              // 5*n int32
-    uint32_t* bnd;             // strip boundary rows, kPairsPerWave*bnd_rows per workgroup
+    void* bnd;                 // strip boundary rows, kPairsPerWave*bnd_rows entries per workgroup
     int64_t bnd_rows;
     int32_t* status;
+    int32_t* redo;             // pairs whose int16 score may have saturated (count: status[kStatusSaturated])
     const uint8_t* lut;        // byte -> residue code, 0xFF for bytes outside the alphabet
     const int16_t* sub;        // A*A scores, index row_code*A + col_code
     const int16_t* colmax;     // per column code: max(0, max_a sub[a*A + code])
@@ -61,7 +62,8 @@ __host__ __device__ constexpr int32_t kremstride(int32_t K)
 // Profile columns reserved per virtual lane.
 __host__ __device__ constexpr int32_t kstride(int32_t K) { return K / 8 * 8 + kremstride(K); }
 
-__host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha)
+// bnd_bytes: bytes of one strip-boundary entry (4 for packed int16, 8 for int32).
+__host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha, int32_t bnd_bytes)
 {
     LdsLayout L;
     L.lut_off = 0;                                       // 256 B
@@ -73,7 +75,7 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha)
     L.prof_bytes = align_up((alpha + 1) * W * 2, 128);
     L.ring_code_off = L.prof_bytes;
     L.ring_bnd_off = L.ring_code_off + kRing * 2;
-    L.group_bytes = align_up(L.ring_bnd_off + kRing * 4, 128);
+    L.group_bytes = align_up(L.ring_bnd_off + kRing * bnd_bytes, 128);
     L.total = L.group_off + kPairsPerWave * L.group_bytes;
     return L;
 }

@@ -62,33 +62,120 @@ __device__ __forceinline__ uint32_t inv_prio(uint32_t p)
     return tie_prio(p);
 }
 
-// Raw bytes of 2 ring rows prefetched 32 steps before they are stored in LDS.
-struct RowFetch {
-    int32_t b0, b1;     // residue byte, -1 outside [0, nr)
-    uint32_t d0, d1;    // strip boundary (H | E << 16)
+// ---------------------------------------------------------------- arithmetic
+
+// Packed16: one VGPR holds the cells of both virtual lanes of a lane as int16
+// halves (low: virtual lane l, high: l+16); every op advances two cells.
+struct Packed16 {
+    using V = uint32_t;
+    using B = uint32_t; // strip boundary entry: H | E << 16
+    static constexpr bool kWide = false;
+    __device__ static V zero() { return 0u; }
+    __device__ static V splat(int32_t x) { return pack16(x, x); }
+    __device__ static V rows_init(int v_lo, int v_hi) { return pack16(-v_lo, -v_hi); }
+    __device__ static V rows_next(V r) { return raw(as_u(r) + (u16x2){1, 1}); }
+    __device__ static int32_t row_lo(V r) { return (int32_t)(r & 0xFFFFu); }
+    __device__ static int32_t row_hi(V r) { return (int32_t)(r >> 16); }
+    __device__ static V dpp(V x) { return row_ror1(x); }
+    // virtual lane 0 takes the strip boundary, virtual lane 16 the low half of lane 15
+    __device__ static V fix_h(V x, B bnd, uint32_t m) { return bfi(m, __builtin_amdgcn_perm(x, bnd, 0x05040100u), x); }
+    __device__ static V fix_e(V x, B bnd, uint32_t m) { return bfi(m, __builtin_amdgcn_perm(x, bnd, 0x05040302u), x); }
+    __device__ static B bnd_out(V h, V e) { return __builtin_amdgcn_perm(e, h, 0x07060302u); } // high halves
+    template <int K>
+    __device__ static V prof(const uint32_t* plo, const uint32_t* phi, int k)
+    {
+        return (k & 1) ? __builtin_amdgcn_perm(phi[k / 2], plo[k / 2], 0x07060302u)
+                       : __builtin_amdgcn_perm(phi[k / 2], plo[k / 2], 0x05040100u);
+    }
+    __device__ static V add(V a, V b) { return pk_add_sat(a, b); }
+    __device__ static V vmax(V a, V b) { return pk_max(a, b); }
+    __device__ static V subf0(V a, V b) { return pk_sub_floor0(a, b); }
+    __device__ static V ltmask(V a, V b) { return pk_lt_mask(a, b); }
+    __device__ static V sel(V m, V a, V b) { return bfi(m, a, b); }
+    __device__ static int32_t lo(V x) { return lo16(x); }
+    __device__ static int32_t hi(V x) { return hi16(x); }
+    __device__ static B load_bnd(const B* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    __device__ static B zero_bnd() { return 0u; }
 };
 
-__device__ __forceinline__ uint32_t load_coherent(const uint32_t* p)
-{
-    // The boundary rows were stored by this workgroup during the previous strip;
-    // an agent-scope load does not hit a stale L1 line.
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Wide32: the same two virtual lanes as two int32 VGPRs, for pairs whose score
+// may exceed the int16 range.
+struct Wide32 {
+    struct V {
+        int32_t l, h;
+    };
+    using B = uint2; // (H, E)
+    static constexpr bool kWide = true;
+    __device__ static V zero() { return V{0, 0}; }
+    __device__ static V splat(int32_t x) { return V{x, x}; }
+    __device__ static V rows_init(int v_lo, int v_hi) { return V{-v_lo, -v_hi}; }
+    __device__ static V rows_next(V r) { return V{r.l + 1, r.h + 1}; }
+    __device__ static int32_t row_lo(V r) { return r.l; }
+    __device__ static int32_t row_hi(V r) { return r.h; }
+    __device__ static V dpp(V x) { return V{(int32_t)row_ror1((uint32_t)x.l), (int32_t)row_ror1((uint32_t)x.h)}; }
+    __device__ static V fix_h(V x, B bnd, uint32_t m)
+    {
+        return V{(int32_t)bfi(m, bnd.x, (uint32_t)x.l), (int32_t)bfi(m, (uint32_t)x.l, (uint32_t)x.h)};
+    }
+    __device__ static V fix_e(V x, B bnd, uint32_t m)
+    {
+        return V{(int32_t)bfi(m, bnd.y, (uint32_t)x.l), (int32_t)bfi(m, (uint32_t)x.l, (uint32_t)x.h)};
+    }
+    __device__ static B bnd_out(V h, V e) { return make_uint2((uint32_t)h.h, (uint32_t)e.h); }
+    template <int K>
+    __device__ static V prof(const uint32_t* plo, const uint32_t* phi, int k)
+    {
+        const int sh = (k & 1) * 16;
+        return V{(int32_t)(int16_t)(plo[k / 2] >> sh), (int32_t)(int16_t)(phi[k / 2] >> sh)};
+    }
+    __device__ static V add(V a, V b) { return V{a.l + b.l, a.h + b.h}; }
+    __device__ static V vmax(V a, V b) { return V{max(a.l, b.l), max(a.h, b.h)}; }
+    __device__ static V subf0(V a, V b)
+    {
+        return V{(int32_t)__builtin_elementwise_sub_sat((uint32_t)a.l, (uint32_t)b.l),
+                 (int32_t)__builtin_elementwise_sub_sat((uint32_t)a.h, (uint32_t)b.h)};
+    }
+    __device__ static V ltmask(V a, V b) { return V{a.l < b.l ? -1 : 0, a.h < b.h ? -1 : 0}; }
+    __device__ static V sel(V m, V a, V b)
+    {
+        return V{(int32_t)bfi((uint32_t)m.l, (uint32_t)a.l, (uint32_t)b.l),
+                 (int32_t)bfi((uint32_t)m.h, (uint32_t)a.h, (uint32_t)b.h)};
+    }
+    __device__ static int32_t lo(V x) { return x.l; }
+    __device__ static int32_t hi(V x) { return x.h; }
+    __device__ static B load_bnd(const B* p)
+    {
+        const uint64_t v = __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+    }
+    __device__ static B zero_bnd() { return make_uint2(0u, 0u); }
+};
 
-__device__ __forceinline__ RowFetch row_fetch(const Task& t, int chunk, int l, const uint32_t* bnd, bool use_bnd)
+// Raw bytes of 2 ring rows prefetched 32 steps before they are stored in LDS.
+template <class A>
+struct RowFetch {
+    int32_t b0, b1;         // residue byte, -1 outside [0, nr)
+    typename A::B d0, d1;   // strip boundary (H, E)
+};
+
+template <class A>
+__device__ __forceinline__ RowFetch<A> row_fetch(const Task& t, int chunk, int l, const typename A::B* bnd, bool use_bnd)
 {
-    RowFetch f;
+    RowFetch<A> f;
     const int x0 = chunk * kChunk + l, x1 = x0 + 16;
     const bool v0 = x0 >= 0 && x0 < t.nr, v1 = x1 >= 0 && x1 < t.nr;
     f.b0 = v0 ? (int32_t)t.rows[t.rev ? t.nr_full - 1 - x0 : x0] : -1;
     f.b1 = v1 ? (int32_t)t.rows[t.rev ? t.nr_full - 1 - x1 : x1] : -1;
-    f.d0 = (v0 && use_bnd) ? load_coherent(bnd + x0) : 0u;
-    f.d1 = (v1 && use_bnd) ? load_coherent(bnd + x1) : 0u;
+    // the boundary rows were stored by this workgroup during the previous strip:
+    // an agent-scope load does not hit a stale L1 line
+    f.d0 = (v0 && use_bnd) ? A::load_bnd(bnd + x0) : A::zero_bnd();
+    f.d1 = (v1 && use_bnd) ? A::load_bnd(bnd + x1) : A::zero_bnd();
     return f;
 }
 
-__device__ __forceinline__ void row_store(const RowFetch& f, int chunk, int l, uint16_t* ring_code,
-                                          uint32_t* ring_bnd, const uint8_t* lut, int alpha, int rowstride)
+template <class A>
+__device__ __forceinline__ void row_store(const RowFetch<A>& f, int chunk, int l, uint16_t* ring_code,
+                                          typename A::B* ring_bnd, const uint8_t* lut, int alpha, int rowstride)
 {
     const int x0 = chunk * kChunk + l, x1 = x0 + 16;
     int c0 = f.b0 < 0 ? alpha : (int)lut[f.b0];
@@ -143,16 +230,16 @@ __device__ __forceinline__ void prof_slot(int j, int& v, int& k)
 }
 
 // Max over K registers as a balanced tree (depth log2 K instead of a chain).
-template <int K>
-__device__ __forceinline__ uint32_t tree_max(const uint32_t (&h)[K])
+template <class A, int K>
+__device__ __forceinline__ typename A::V tree_max(const typename A::V (&h)[K])
 {
-    uint32_t t[K];
+    typename A::V t[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) t[k] = h[k];
 #pragma unroll
     for (int w = 1; w < K; w *= 2)
 #pragma unroll
-        for (int k = 0; k + w < K; k += 2 * w) t[k] = pk_max(t[k], t[k + w]);
+        for (int k = 0; k + w < K; k += 2 * w) t[k] = A::vmax(t[k], t[k + w]);
     return t[0];
 }
 
@@ -166,10 +253,12 @@ struct PassResult {
 //
 // Forward key: (score << 40) | ((2^20-1-row) << 20) | (2^20-1-prio(col)), max wins.
 // Reverse key: ((2^20-1-col) << 20) | (2^20-1-row), max wins (target = score).
-template <int K, bool REV>
+template <class A, int K, bool REV>
 __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& t, int32_t target,
-                               uint32_t* bnd_g)
+                               typename A::B* bnd_g)
 {
+    using V = typename A::V;
+    using Bd = typename A::B;
     constexpr int KS = kstride(K);
     constexpr int W = kVirtPerPair * K;   // columns of one strip
     constexpr int WA = kVirtPerPair * KS; // profile columns reserved per row
@@ -181,7 +270,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     uint8_t* gb = sm.group(g);
     int16_t* prof = (int16_t*)gb;
     uint16_t* ring_code = (uint16_t*)(gb + sm.L.ring_code_off);
-    uint32_t* ring_bnd = (uint32_t*)(gb + sm.L.ring_bnd_off);
+    Bd* ring_bnd = (Bd*)(gb + sm.L.ring_bnd_off);
     const uint8_t* lut = sm.lut();
     const int16_t* sub = sm.sub();
 
@@ -200,8 +289,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     }
 
     const bool multi_strip = strips_max > 1;
-    const uint32_t OPEN2 = pack16(-a.gap_open, -a.gap_open);
-    const uint32_t EXT2 = pack16(-a.gap_extend, -a.gap_extend);
+    const V OPEN2 = A::splat(-a.gap_open);
+    const V EXT2 = A::splat(-a.gap_extend);
     const int v_lo = l, v_hi = l + 16;
     uint64_t cand_lo = 0, cand_hi = 0;
 
@@ -227,32 +316,34 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         }
         // ---- row ring: rows [-32, 0) padding, [0, 32) now, [32, 64) in flight
         {
-            RowFetch pad; pad.b0 = pad.b1 = -1; pad.d0 = pad.d1 = 0;
-            row_store(pad, -1, l, ring_code, ring_bnd, lut, alpha, rowstride);
-            RowFetch f0 = row_fetch(t, 0, l, bnd_g, use_bnd);
-            row_store(f0, 0, l, ring_code, ring_bnd, lut, alpha, rowstride);
+            RowFetch<A> pad;
+            pad.b0 = pad.b1 = -1;
+            pad.d0 = pad.d1 = A::zero_bnd();
+            row_store<A>(pad, -1, l, ring_code, ring_bnd, lut, alpha, rowstride);
+            RowFetch<A> f0 = row_fetch<A>(t, 0, l, bnd_g, use_bnd);
+            row_store<A>(f0, 0, l, ring_code, ring_bnd, lut, alpha, rowstride);
         }
-        RowFetch pf = row_fetch(t, 1, l, bnd_g, use_bnd);
+        RowFetch<A> pf = row_fetch<A>(t, 1, l, bnd_g, use_bnd);
         __syncthreads();
 
 
-        uint32_t Hcol[K], Fcol[K];
-        uint32_t snap[REV ? 1 : K];
+        V Hcol[K], Fcol[K];
+        V snap[REV ? 1 : K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) { Hcol[k] = 0; Fcol[k] = 0; }
+        for (int k = 0; k < K; ++k) { Hcol[k] = A::zero(); Fcol[k] = A::zero(); }
 #pragma unroll
-        for (int k = 0; k < (REV ? 1 : K); ++k) snap[k] = 0;
-        uint32_t Hout = 0, Eout = 0, Hdiag0 = 0;
-        uint32_t best = 0, brow = 0;
-        uint32_t rowpk = pack16(-v_lo, -v_hi); // row of each half at step 0
+        for (int k = 0; k < (REV ? 1 : K); ++k) snap[k] = A::zero();
+        V Hout = A::zero(), Eout = A::zero(), Hdiag0 = A::zero();
+        V best = A::zero(), brow = A::zero();
+        V rowpk = A::rows_init(v_lo, v_hi); // row of each half at step 0
         int32_t rt_lo = 0x7FFFFFFF, rr_lo = 0, rt_hi = 0x7FFFFFFF, rr_hi = 0; // reverse hits
 
         for (int s0 = 0; s0 < steps_max; s0 += kChunk) {
             // refill: rows of chunk (s0/32 + 1) enter the ring, chunk +2 is requested
             {
                 const int ck = s0 / kChunk + 1;
-                row_store(pf, ck, l, ring_code, ring_bnd, lut, alpha, rowstride);
-                pf = row_fetch(t, ck + 1, l, bnd_g, use_bnd);
+                row_store<A>(pf, ck, l, ring_code, ring_bnd, lut, alpha, rowstride);
+                pf = row_fetch<A>(t, ck + 1, l, bnd_g, use_bnd);
             }
             __syncthreads();
             // two steps per iteration (manual unroll: the DPP moves are convergent,
@@ -262,27 +353,26 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             auto step = [&](const int s) {
                 // left inputs: neighbour's outputs of the previous step; virtual lane 0
                 // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
-                uint32_t xH = row_ror1(Hout), xE = row_ror1(Eout);
-                const uint32_t bnd = multi_strip ? ring_bnd[s & (kRing - 1)] : 0u;
-                xH = bfi(lane0_mask, __builtin_amdgcn_perm(xH, bnd, 0x05040100u), xH);
-                xE = bfi(lane0_mask, __builtin_amdgcn_perm(xE, bnd, 0x05040302u), xE);
+                V xH = A::dpp(Hout), xE = A::dpp(Eout);
+                const Bd bnd = multi_strip ? ring_bnd[s & (kRing - 1)] : A::zero_bnd();
+                xH = A::fix_h(xH, bnd, lane0_mask);
+                xE = A::fix_e(xE, bnd, lane0_mask);
                 const uint32_t off_lo = ring_code[(s - v_lo) & (kRing - 1)];
                 const uint32_t off_hi = ring_code[(s - v_hi) & (kRing - 1)];
                 uint32_t plo[KS / 2], phi[KS / 2];
                 prof_load<K>((const uint8_t*)prof + off_lo, v_lo, plo);
                 prof_load<K>((const uint8_t*)prof + off_hi, v_hi, phi);
 
-                uint32_t E = xE, Hd = Hdiag0;
+                V E = xE, Hd = Hdiag0;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const uint32_t S = (k & 1) ? __builtin_amdgcn_perm(phi[k / 2], plo[k / 2], 0x07060302u)
-                                               : __builtin_amdgcn_perm(phi[k / 2], plo[k / 2], 0x05040100u);
-                    const uint32_t D = pk_add_sat(Hd, S);
+                    const V S = A::template prof<K>(plo, phi, k);
+                    const V D = A::add(Hd, S);
                     Hd = Hcol[k];
-                    const uint32_t H = pk_max(pk_max(D, E), Fcol[k]);
-                    const uint32_t Ho = pk_sub_floor0(H, OPEN2);
-                    E = pk_max(pk_sub_floor0(E, EXT2), Ho);
-                    Fcol[k] = pk_max(pk_sub_floor0(Fcol[k], EXT2), Ho);
+                    const V H = A::vmax(A::vmax(D, E), Fcol[k]);
+                    const V Ho = A::subf0(H, OPEN2);
+                    E = A::vmax(A::subf0(E, EXT2), Ho);
+                    Fcol[k] = A::vmax(A::subf0(Fcol[k], EXT2), Ho);
                     Hcol[k] = H;
                 }
                 Hout = Hcol[K - 1];
@@ -293,25 +383,25 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 if (multi_strip) {
                     const int row = s - v_hi;
                     if (write_bnd && l == 15 && row >= 0 && row < t.nr)
-                        bnd_g[row] = __builtin_amdgcn_perm(Eout, Hout, 0x07060302u);
+                        bnd_g[row] = A::bnd_out(Hout, Eout);
                 }
 
-                const uint32_t rm = tree_max<K>(Hcol);
+                const V rm = tree_max<A, K>(Hcol);
                 if constexpr (!REV) {
-                    const uint32_t m = pk_lt_mask(best, rm);
-                    best = pk_max(best, rm);
-                    brow = bfi(m, rowpk, brow);
+                    const V m = A::ltmask(best, rm);
+                    best = A::vmax(best, rm);
+                    brow = A::sel(m, rowpk, brow);
 #pragma unroll
-                    for (int k = 0; k < K; ++k) snap[k] = bfi(m, Hcol[k], snap[k]);
+                    for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
                 } else {
-                    if (lo16(rm) == target || hi16(rm) == target) {
+                    if (A::lo(rm) == target || A::hi(rm) == target) {
                         const int row_lo = s - v_lo, row_hi = s - v_hi;
                         // first (smallest) column of this row that reaches the target
                         int k_lo = K, k_hi = K;
 #pragma unroll
                         for (int k = K - 1; k >= 0; --k) {
-                            if (lo16(Hcol[k]) == target && strip_base + v_lo * K + k < t.nc) k_lo = k;
-                            if (hi16(Hcol[k]) == target && strip_base + v_hi * K + k < t.nc) k_hi = k;
+                            if (A::lo(Hcol[k]) == target && strip_base + v_lo * K + k < t.nc) k_lo = k;
+                            if (A::hi(Hcol[k]) == target && strip_base + v_hi * K + k < t.nc) k_hi = k;
                         }
                         if (k_lo < K && row_lo >= 0 && row_lo < t.nr && strip_base + v_lo * K + k_lo < rt_lo) {
                             rt_lo = strip_base + v_lo * K + k_lo;
@@ -323,7 +413,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                         }
                     }
                 }
-                rowpk = raw(as_u(rowpk) + (u16x2){1, 1});
+                rowpk = A::rows_next(rowpk);
             };
             for (int s = s0; s < s_end; s += 2) {
                 step(s);
@@ -335,14 +425,14 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         if constexpr (!REV) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int32_t b = h ? hi16(best) : lo16(best);
+                const int32_t b = h ? A::hi(best) : A::lo(best);
                 if (live && b > 0) {
-                    const int32_t row = (int32_t)(h ? (brow >> 16) : (brow & 0xFFFFu));
+                    const int32_t row = h ? A::row_hi(brow) : A::row_lo(brow);
                     const int base = strip_base + (h ? v_hi : v_lo) * K;
                     uint32_t pbest = 0xFFFFFFFFu;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const int32_t v = h ? hi16(snap[k]) : lo16(snap[k]);
+                        const int32_t v = h ? A::hi(snap[k]) : A::lo(snap[k]);
                         const int col = base + k;
                         if (v == b && col < t.nc) pbest = min(pbest, tie_prio((uint32_t)col));
                     }
@@ -421,7 +511,9 @@ __device__ __forceinline__ Task forward_task(const KernelArgs& a, int64_t p)
     return t;
 }
 
-__device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p, uint64_t key, int32_t* status)
+// Decodes the forward key.  A score at or above sat_limit may have saturated the
+// int16 cells: the pair is queued for the int32 kernel and reported as score -1.
+__device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p, uint64_t key)
 {
     FwdOut o{0, 0, 0};
     if (p < 0 || key == 0) return o;
@@ -430,10 +522,17 @@ __device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p,
     const int32_t score = (int32_t)(key >> 40);
     const int32_t row = (int32_t)(0xFFFFFu - (uint32_t)((key >> 20) & 0xFFFFFu));
     const int32_t col = (int32_t)inv_prio(0xFFFFFu - (uint32_t)(key & 0xFFFFFu));
+    if (score >= a.sat_limit) {
+        if ((threadIdx.x & 15) == 0) {
+            const int32_t slot = atomicAdd(a.status + kStatusSaturated, 1);
+            a.redo[slot] = (int32_t)p;
+        }
+        o.score = -1;
+        return o;
+    }
     o.score = score;
     if (la < lb) { o.query_end = row + 1; o.ref_end = col + 1; } // kernel.cpp:381-386
     else { o.ref_end = row + 1; o.query_end = col + 1; }        // kernel.cpp:387-392
-    if (score >= a.sat_limit) atomicOr(status + kStatusSaturated, 1);
     return o;
 }
 
@@ -527,22 +626,35 @@ __device__ __forceinline__ int64_t slot_pair(const KernelArgs& a, int64_t slot)
 
 // ---------------------------------------------------------------- kernels
 
+template <class A, int K>
+__device__ __forceinline__ Smem make_smem(uint8_t* raw, const KernelArgs& a)
+{
+    return Smem{raw, lds_layout(kVirtPerPair * kstride(K), a.alpha, (int)sizeof(typename A::B))};
+}
+
+template <class A>
+__device__ __forceinline__ typename A::B* group_bnd(const KernelArgs& a)
+{
+    const int g = threadIdx.x >> 4;
+    return (typename A::B*)a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
+}
+
 // Plain form, forward pass: quads of consecutive pairs in input order, statically
 // strided over the grid (one launch per batch, like driver.cpp:176-185).
 template <int K>
 __global__ __launch_bounds__(64) void k_plain_forward(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    Smem sm{smem_raw, lds_layout(kVirtPerPair * kstride(K), a.alpha)};
+    const Smem sm = make_smem<Packed16, K>(smem_raw, a);
     stage_tables(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
+    Packed16::B* bnd_g = group_bnd<Packed16>(a);
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
     for (int64_t q = blockIdx.x; q < nquads; q += gridDim.x) {
         const int64_t p = slot_pair(a, q * kPairsPerWave + g);
         const Task t = forward_task(a, p);
-        const PassResult r = run_pass<K, false>(a, sm, t, 0, bnd_g);
-        const FwdOut f = forward_result(a, p, r.key, a.status);
+        const PassResult r = run_pass<Packed16, K, false>(a, sm, t, 0, bnd_g);
+        const FwdOut f = forward_result(a, p, r.key);
         if (p >= 0 && l == 0) {
             a.out[kOutScore * a.n + p] = f.score;
             a.out[kOutRefEnd * a.n + p] = f.ref_end;
@@ -557,10 +669,10 @@ template <int K>
 __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    Smem sm{smem_raw, lds_layout(kVirtPerPair * kstride(K), a.alpha)};
+    const Smem sm = make_smem<Packed16, K>(smem_raw, a);
     stage_tables(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
+    Packed16::B* bnd_g = group_bnd<Packed16>(a);
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
     for (int64_t q = blockIdx.x; q < nquads; q += gridDim.x) {
         const int64_t p = slot_pair(a, q * kPairsPerWave + g);
@@ -571,10 +683,10 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
             f.query_end = a.out[kOutQueryEnd * a.n + p];
         }
         const Task t = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult r = run_pass<K, true>(a, sm, t, f.score, bnd_g);
+        const PassResult r = run_pass<Packed16, K, true>(a, sm, t, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, t, r.key, &rb, &qb);
-        if (p >= 0 && l == 0) {
+        if (p >= 0 && l == 0 && f.score >= 0) {
             a.out[kOutRefBegin * a.n + p] = rb;
             a.out[kOutQueryBegin * a.n + p] = qb;
         }
@@ -582,15 +694,17 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
 }
 
 // Batched form: persistent workgroups pull quads of length-sorted pairs from a
-// device work queue and run forward then reverse back to back.
-template <int K>
+// device work queue and run forward then reverse back to back.  With A = Wide32
+// it recomputes the pairs the int16 kernels queued as possibly saturated.
+template <class A, int K>
 __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    Smem sm{smem_raw, lds_layout(kVirtPerPair * kstride(K), a.alpha)};
+    const Smem sm = make_smem<A, K>(smem_raw, a);
     stage_tables(a, sm);
-    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    uint32_t* bnd_g = a.bnd + ((int64_t)blockIdx.x * kPairsPerWave + g) * a.bnd_rows;
+    const int l = threadIdx.x & 15;
+    const int g = threadIdx.x >> 4;
+    typename A::B* bnd_g = group_bnd<A>(a);
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
     for (;;) {
         uint32_t q = 0;
@@ -599,13 +713,13 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
         if ((int64_t)q >= nquads) break;
         const int64_t p = slot_pair(a, (int64_t)q * kPairsPerWave + g);
         const Task tf = forward_task(a, p);
-        const PassResult rf = run_pass<K, false>(a, sm, tf, 0, bnd_g);
-        const FwdOut f = forward_result(a, p, rf.key, a.status);
+        const PassResult rf = run_pass<A, K, false>(a, sm, tf, 0, bnd_g);
+        const FwdOut f = forward_result(a, p, rf.key);
         const Task tr = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult rr = run_pass<K, true>(a, sm, tr, f.score, bnd_g);
+        const PassResult rr = run_pass<A, K, true>(a, sm, tr, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, tr, rr.key, &rb, &qb);
-        if (p >= 0 && l == 0) {
+        if (p >= 0 && l == 0 && f.score >= 0) {
             a.out[kOutScore * a.n + p] = f.score;
             a.out[kOutRefEnd * a.n + p] = f.ref_end;
             a.out[kOutQueryEnd * a.n + p] = f.query_end;
@@ -629,14 +743,31 @@ static const void* kernel_ptr(int kind)
     switch (kind) {
     case kPlainForward: return (const void*)&k_plain_forward<K>;
     case kPlainReverse: return (const void*)&k_plain_reverse<K>;
-    default: return (const void*)&k_batched<K>;
+    case kBatched: return (const void*)&k_batched<Packed16, K>;
+    default: return nullptr;
     }
+}
+
+template <int K>
+static const void* wide_ptr()
+{
+    return (const void*)&k_batched<Wide32, K>;
 }
 
 #define GBSW_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
 
+#define GBSW_WIDE_K_LIST(X) X(4) X(8) X(16)
+
 static const void* kernel_for(int kind, int K)
 {
+    if (kind == kBatchedWide) {
+        switch (K) {
+#define GBSW_CASE(k) case k: return wide_ptr<k>();
+            GBSW_WIDE_K_LIST(GBSW_CASE)
+#undef GBSW_CASE
+        default: return nullptr;
+        }
+    }
     switch (K) {
 #define GBSW_CASE(k) case k: return kernel_ptr<k>(kind);
         GBSW_K_LIST(GBSW_CASE)
@@ -646,6 +777,17 @@ static const void* kernel_for(int kind, int K)
 }
 
 bool kernel_supported(int K) { return kernel_for(kPlainForward, K) != nullptr; }
+
+const int* wide_k_list(int* count)
+{
+    static const int ks[] = {
+#define GBSW_ITEM(k) k,
+        GBSW_WIDE_K_LIST(GBSW_ITEM)
+#undef GBSW_ITEM
+    };
+    *count = (int)(sizeof(ks) / sizeof(ks[0]));
+    return ks;
+}
 
 const int* kernel_k_list(int* count)
 {

@@ -6,10 +6,11 @@
 
 namespace gbsw {
 
-enum KernelKind { kPlainForward = 0, kPlainReverse = 1, kBatched = 2 };
+enum KernelKind { kPlainForward = 0, kPlainReverse = 1, kBatched = 2, kBatchedWide = 3 };
 
 bool kernel_supported(int K);
 const int* kernel_k_list(int* count); // K values with compiled kernels, ascending
+const int* wide_k_list(int* count);   // K values of the int32 (kBatchedWide) kernel
 int kernel_occupancy(int kind, int K, int lds_bytes);
 hipError_t kernel_launch(int kind, int K, int grid, int lds_bytes, hipStream_t stream, const KernelArgs& args);
 hipError_t zero_u32(uint32_t* p, int n, hipStream_t stream);

@@ -192,3 +192,37 @@ def test_drop_in_python_driver(ctx):
             int(r["query_begin"][0]), int(r["query_end"][0])) == (4, 5, 9, 5, 9)
     a = gbsw.kernel_driver_aa(["MKV"], ["AMKVA"], BLOSUM62, -6, -1)
     assert int(a["score"][0]) == 5 + 5 + 4
+
+
+def test_config3_long_protein_full_length(ctx):
+    """Long-protein pairs at full length (query 1024-4096, ref 4096-16384):
+    several strips, int16 cells with saturation detection."""
+    w = synth.long_protein(48, seed=303)
+    run_both(ctx, w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring, "config3_48")
+
+
+def test_int16_saturation_falls_back_to_int32(ctx):
+    """Scores beyond the int16 range: identical W-rich proteins (11 per residue)
+    and DNA with a large match score, mixed with ordinary pairs.  Saturated pairs
+    are recomputed with int32 cells and must still match the oracle exactly."""
+    rng = random.Random(17)
+    refs, qrys = [], []
+    for k in range(24):
+        if k % 3 == 0:
+            core = "".join(rng.choice("WWWWC") for _ in range(rng.randint(3000, 4200)))
+            refs.append("AKR" * 50 + core + "GG" * 40)
+            qrys.append(core[: len(core) - rng.randint(0, 50)])
+        else:
+            refs.append("".join(rng.choice("ARNDCQEGHILKMFPSTWYV") for _ in range(rng.randint(200, 900))))
+            qrys.append(refs[-1][rng.randint(0, 50):][:rng.randint(100, 400)])
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    exp = run_both(ctx, rb, ro, qb, qo, gbsw.PROTEIN, (-6, -1), "saturation_protein")
+    assert exp["score"].max() > 32767
+    refs, qrys = rand_pairs(18, 30, 300, 900, ["ACGT"])
+    for k in range(0, 30, 5):
+        qrys[k] = refs[k][10:10 + 700] if len(refs[k]) > 720 else refs[k]
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    exp = run_both(ctx, rb, ro, qb, qo, gbsw.DNA, (100, -3, -6, -1), "saturation_dna")
+    assert exp["score"].max() > 32767
