@@ -11,10 +11,15 @@ Only data files are read (no reference code is imported or run):
 Output:
   protein_query_golden.tsv : first N records as "<query>\t<golden score>"
   dna_golden_scores.txt    : the 30,000 DNA golden scores (inputs are not shipped)
+  reference_tables.json    : the numbers of the reference's residue encoding
+                             (driver.cpp:252-258) and of the BLOSUM62 matrix its
+                             sample program passes (main.cpp:70-92), parsed as data
 The protein reference sequences and all DNA inputs are missing from the
 reference checkout (.MISSING_LARGE_BLOBS), so the goldens can only bound scores.
 """
+import json
 import os
+import re
 import sys
 
 N_PROTEIN = 3000
@@ -46,7 +51,17 @@ def main(ref_root):
         dna = [int(x.split()[0]) for x in fh if x.strip()]
     with open(os.path.join(here, "dna_golden_scores.txt"), "w") as out:
         out.write("\n".join(str(v) for v in dna) + "\n")
-    print("wrote %d protein and %d DNA golden rows" % (N_PROTEIN, len(dna)))
+    drv = open(os.path.join(ref_root, "driver.cpp")).read()
+    a = drv.index("short encoding_matrix[] = {")
+    enc = [int(x) for x in re.findall(r"-?\d+", drv[drv.index("{", a) + 1:drv.index("}", a)])]
+    main = open(os.path.join(ref_root, "main.cpp")).read()
+    b = main.index("short scores_matrix[] = {")
+    blosum = [int(x) for x in re.findall(r"-?\d+", main[main.index("{", b) + 1:main.index("}", b)])]
+    assert len(enc) == 91 and len(blosum) == 576
+    with open(os.path.join(here, "reference_tables.json"), "w") as out:
+        json.dump({"source": "numbers parsed from driver.cpp:252-258 (encoding_matrix) and main.cpp:70-92 "
+                             "(scores_matrix) of the reference", "encoding": enc, "blosum62": blosum}, out)
+    print("wrote %d protein and %d DNA golden rows, reference tables" % (N_PROTEIN, len(dna)))
 
 
 if __name__ == "__main__":

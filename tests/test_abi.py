@@ -61,3 +61,18 @@ def test_pack_roundtrip():
     buf, off = gbsw.pack(["AC", "", "GGT"])
     assert list(off) == [0, 2, 2, 5]
     assert bytes(buf[:5]) == b"ACGGT"
+
+
+def build_dropin_caller(tmpdir):
+    """Compile a GPU-BSW-style caller against include/ and link libgbsw.so."""
+    exe = os.path.join(str(tmpdir), "dropin_caller")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(REPO, "include"),
+                    os.path.join(REPO, "tests", "dropin", "dropin_caller.cpp"),
+                    "-L", os.path.dirname(gbsw.library_path()), "-lgbsw",
+                    "-Wl,-rpath," + os.path.dirname(gbsw.library_path()), "-o", exe], check=True)
+    return exe
+
+
+def test_dropin_caller_compiles_and_links(tmp_path):
+    exe = build_dropin_caller(tmp_path)
+    assert os.path.exists(exe)

@@ -415,3 +415,15 @@ def test_reverse_extent_bound_is_exact():
             assert (rb, qb) == ((orow, ocol) if rev_rows_ref else (ocol, orow))
             checked += 1
         assert checked > 200
+
+
+def test_tables_match_reference_data():
+    """The oracle's residue encoding and the host package's BLOSUM62 equal the
+    numbers of the reference's own tables (driver.cpp:252-258, main.cpp:70-92)."""
+    import json
+    with open(os.path.join(HERE, "golden", "reference_tables.json")) as fh:
+        ref = json.load(fh)
+    assert [ol.encode_aa(b) for b in range(91)] == ref["encoding"]
+    assert [encode_aa(b) for b in range(91)] == ref["encoding"]
+    assert BLOSUM62 == ref["blosum62"]
+    assert ol.encode_aa(91) == -1 and ol.encode_aa(-1) == -1
