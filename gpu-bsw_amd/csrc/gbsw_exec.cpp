@@ -1,0 +1,436 @@
+// gbsw_exec.cpp -- the alignment pipeline of one job: geometry choice, length
+// buckets, device binning, kernel launches on concurrent streams, the int32
+// recompute of saturated pairs, and timing.
+//
+// Replaces the per-chunk body of driver.cpp:88-221 (pack, two-stream copies,
+// forward launch, host newMin, reverse launch, copies back).
+#include <algorithm>
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "gbsw_internal.hpp"
+
+namespace gbsw {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+const char* last_error_text() { return g_err.c_str(); }
+
+void DeviceBuffer::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+hipError_t DeviceBuffer::reserve(size_t b)
+{
+    if (b <= bytes) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e == hipSuccess) bytes = b;
+    else ptr = nullptr;
+    return e;
+}
+
+void PinnedBuffer::release()
+{
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+hipError_t PinnedBuffer::reserve(size_t b)
+{
+    if (b <= bytes) return hipSuccess;
+    release();
+    hipError_t e = hipHostMalloc(&ptr, b, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = b;
+    else ptr = nullptr;
+    return e;
+}
+
+namespace {
+
+// Streams used for the length buckets: 1 + kAuxStreams by default;
+// GBSW_STREAMS=1 serialises the buckets (per-kernel timing and counters).
+int bucket_streams()
+{
+    static const int n = [] {
+        const char* e = std::getenv("GBSW_STREAMS");
+        const int v = e ? std::atoi(e) : kAuxStreams + 1;
+        return std::max(1, std::min(kAuxStreams + 1, v));
+    }();
+    return n;
+}
+
+// LDS bytes of a workgroup; bnd_bytes = 4 (packed int16) or 8 (int32 kernel).
+int lds_for(int K, int alpha, int bnd_bytes = 4)
+{
+    return lds_layout(kVirtPerPair * kstride(K), alpha, bnd_bytes).total;
+}
+
+int grid_for(const Exec& ex, int kind, int K, int lds, int64_t nquads)
+{
+    int occ = kernel_occupancy(kind, K, lds);
+    if (occ < 1) occ = 1;
+    const int64_t full = (int64_t)occ * ex.num_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(full, nquads));
+}
+
+// Launch one kernel between two timing events.
+int launch_timed(Exec& ex, hipStream_t stream, int kind, int K, int grid, int lds, KernelArgs& a, void* bnd)
+{
+    const size_t slot = ex.launches.size();
+    a.bnd = bnd;
+    while (ex.kev.size() < 2 * (slot + 1)) {
+        hipEvent_t e;
+        GBSW_HIP_TRY(hipEventCreate(&e));
+        ex.kev.push_back(e);
+    }
+    GBSW_HIP_TRY(hipEventRecord(ex.kev[2 * slot], stream));
+    GBSW_HIP_TRY(kernel_launch(kind, K, grid, lds, stream, a));
+    GBSW_HIP_TRY(hipEventRecord(ex.kev[2 * slot + 1], stream));
+    ex.launches.push_back({kind, K, grid, 0, 0, 0.0});
+    return GBSW_OK;
+}
+
+KernelArgs base_args(const Exec& ex, const Job& job, const gbsw_scoring* sc)
+{
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.ref = job.ref;
+    a.ref_off = job.ref_off;
+    a.qry = job.qry;
+    a.qry_off = job.qry_off;
+    a.n = job.n;
+    a.out = job.out;
+    a.status = ex.status.as<int32_t>();
+    a.redo = ex.redo.as<int32_t>();
+    a.lut = ex.lut.as<uint8_t>();
+    a.sub = ex.sub.as<int16_t>();
+    a.colmax = ex.colmax.as<int16_t>();
+    a.alpha = job.alpha;
+    a.gap_min = std::min(-(int32_t)sc->gap_open, -(int32_t)sc->gap_extend);
+    a.gap_open = sc->gap_open;
+    a.gap_extend = sc->gap_extend;
+    a.sat_limit = 32767 - ex.max_sub;
+    a.bnd_rows = std::max(1, job.max_rows);
+    return a;
+}
+
+} // namespace
+
+int Exec::init(int dev, int ncu)
+{
+    device = dev;
+    num_cu = ncu;
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&ev_stop);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
+    for (int i = 0; i < kAuxStreams && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_join[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = status.reserve(sizeof(int32_t) * kStatusWords);
+    if (e == hipSuccess) e = status_host.reserve(sizeof(int32_t) * kStatusWords);
+    if (e == hipSuccess) e = tables_host.reserve(4096);
+    if (e == hipSuccess) e = queue.reserve(sizeof(uint32_t) * 64);
+    if (e == hipSuccess) e = sub.reserve(sizeof(int16_t) * 1024);
+    if (e == hipSuccess) e = lut.reserve(256);
+    if (e == hipSuccess) e = colmax.reserve(sizeof(int16_t) * 32);
+    if (e != hipSuccess) return fail(GBSW_E_DEVICE, std::string("stream/scratch setup: ") + hipGetErrorString(e));
+    return GBSW_OK;
+}
+
+void Exec::destroy()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int i = 0; i < kAuxStreams; ++i)
+        if (aux[i]) (void)hipStreamSynchronize(aux[i]);
+    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+    kev.clear();
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (int i = 0; i < kAuxStreams; ++i) {
+        if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
+        if (aux[i]) (void)hipStreamDestroy(aux[i]);
+        ev_join[i] = nullptr;
+        aux[i] = nullptr;
+    }
+    bnd.release(); queue.release(); status.release(); sub.release(); lut.release(); colmax.release(); redo.release();
+    status_host.release();
+    tables_host.release();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    ev_start = ev_stop = ev_fork = nullptr;
+}
+
+int max_k_for(int alpha)
+{
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    int best = ks[0];
+    for (int i = 0; i < nk; ++i)
+        if (lds_for(ks[i], alpha) <= 65536) best = ks[i];
+    return best;
+}
+
+int k_for_cols(int cols, int kmax)
+{
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    int best = ks[0];
+    int64_t best_cost = INT64_MAX;
+    for (int i = 0; i < nk; ++i) {
+        const int K = ks[i];
+        if (K > kmax) break;
+        const int64_t strips = std::max<int64_t>(1, (cols + kVirtPerPair * K - 1) / (kVirtPerPair * K));
+        const int64_t cost = strips * (K + 2) * 4 + (strips - 1);
+        if (cost < best_cost) { best_cost = cost; best = K; }
+    }
+    return best;
+}
+
+const std::vector<uint8_t>& bucket_table(int kmax)
+{
+    static std::mutex mu;
+    static std::map<int, std::vector<uint8_t>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(kmax);
+    if (it != cache.end()) return it->second;
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    std::vector<uint8_t> t(65536);
+    for (int m = 0; m < 65536; ++m) {
+        const int K = k_for_cols(m, kmax);
+        int idx = 0;
+        while (idx < nk && ks[idx] != K) ++idx;
+        t[(size_t)m] = (uint8_t)idx;
+    }
+    return cache.emplace(kmax, std::move(t)).first->second;
+}
+
+void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry)
+{
+    const int kmax = max_k_for(job.alpha);
+    const std::vector<uint8_t>& tab = bucket_table(kmax);
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    std::vector<int64_t> count((size_t)nk, 0), cells((size_t)nk, 0);
+    for (int64_t p = 0; p < job.n; ++p) {
+        const int32_t la = len_ref[p], lb = len_qry[p];
+        const int bi = tab[(size_t)std::min<int32_t>(std::min(la, lb), 65535)];
+        count[(size_t)bi] += 1;
+        cells[(size_t)bi] += (int64_t)la * lb;
+    }
+    job.buckets.clear();
+    int64_t at = 0;
+    for (int i = 0; i < nk; ++i) {
+        if (count[(size_t)i] == 0) continue;
+        Bucket bk;
+        bk.K = ks[i];
+        bk.begin = at;
+        bk.count = count[(size_t)i];
+        bk.cells = cells[(size_t)i];
+        at += bk.count;
+        job.buckets.push_back(bk);
+    }
+}
+
+int batched_grid_cap(const Exec& ex, int K, int alpha)
+{
+    const int occ = kernel_occupancy(kBatched, K, lds_for(K, alpha));
+    return std::max(1, occ) * ex.num_cu;
+}
+
+int sort_scratch_bytes(int64_t n, size_t* bytes)
+{
+    size_t tb = 0;
+    GBSW_HIP_TRY(sort_pairs(nullptr, tb, nullptr, nullptr, nullptr, nullptr, (int)std::max<int64_t>(1, n), 20, nullptr));
+    *bytes = std::max<size_t>(tb, 16);
+    return GBSW_OK;
+}
+
+int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub)
+{
+    if (sc->alphabet != job.alphabet) return fail(GBSW_E_ARGUMENT, "scoring alphabet differs from the batch alphabet");
+    if (sc->gap_open > 0 || sc->gap_extend > 0) return fail(GBSW_E_SCORING, "gap_open and gap_extend must be <= 0");
+    const int A = job.alpha;
+    sub.assign((size_t)std::max(1, A * A), 0);
+    int mx = 0;
+    if (job.alphabet == GBSW_DNA) {
+        for (int r = 0; r < A; ++r)
+            for (int c = 0; c < A; ++c) sub[(size_t)(r * A + c)] = r == c ? sc->match : sc->mismatch;
+        mx = std::max<int>(sc->match, sc->mismatch);
+    } else {
+        for (int k = 0; k < 576; ++k) {
+            sub[(size_t)k] = sc->matrix[k];
+            mx = std::max<int>(mx, sc->matrix[k]);
+        }
+    }
+    for (int16_t v : sub)
+        if (v < -16000 || v > 16000) return fail(GBSW_E_SCORING, "substitution scores must lie in [-16000, 16000]");
+    *max_sub = std::max(0, mx);
+    // result keys hold scores in 23 bits; int16 saturation is handled per pair
+    if ((int64_t)job.max_cols * *max_sub >= (1 << 23)) return fail(GBSW_E_RANGE, "possible score exceeds 2^23");
+    return GBSW_OK;
+}
+
+int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
+{
+    ex.launches.clear();
+    std::vector<int16_t> sub;
+    int max_sub = 0;
+    int rc = prepare_scoring(job, sc, sub, &max_sub);
+    if (rc) return rc;
+    ex.max_sub = max_sub;
+    hipStream_t st0 = ex.stream;
+    GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st0));
+    if (job.n == 0) {
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st0));
+        std::memset(ex.status_host.ptr, 0, sizeof(int32_t) * kStatusWords);
+        return GBSW_OK;
+    }
+    std::vector<int16_t> colmax((size_t)job.alpha, 0);
+    for (int c = 0; c < job.alpha; ++c)
+        for (int r = 0; r < job.alpha; ++r)
+            colmax[(size_t)c] = std::max<int16_t>(colmax[(size_t)c], sub[(size_t)(r * job.alpha + c)]);
+    // small tables through pinned staging (the previous job of this Exec has
+    // completed, so its staging is free): the copies stay asynchronous
+    uint8_t* th = ex.tables_host.as<uint8_t>();
+    std::memcpy(th, sub.data(), sizeof(int16_t) * sub.size());
+    std::memcpy(th + 2048, job.lut_host, 256);
+    std::memcpy(th + 2304, colmax.data(), sizeof(int16_t) * colmax.size());
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.sub.ptr, th, sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, st0));
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.lut.ptr, th + 2048, 256, hipMemcpyHostToDevice, st0));
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.colmax.ptr, th + 2304, sizeof(int16_t) * colmax.size(), hipMemcpyHostToDevice, st0));
+    GBSW_HIP_TRY(ex.redo.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(1, job.n)));
+    KernelArgs a = base_args(ex, job, sc);
+    const int kmax = max_k_for(job.alpha);
+    GBSW_HIP_TRY(zero_u32(ex.status.as<uint32_t>(), kStatusWords, st0));
+    if (mode == GBSW_PLAIN) {
+        // one geometry for the whole batch, sized by its longest shorter sequence
+        const int K = k_for_cols(job.max_cols, kmax);
+        const int lds = lds_for(K, job.alpha);
+        const int64_t nquads = (job.n + kPairsPerWave - 1) / kPairsPerWave;
+        a.order = nullptr;
+        a.order_n = job.n;
+        const int gf = grid_for(ex, kPlainForward, K, lds, nquads);
+        const int gr = grid_for(ex, kPlainReverse, K, lds, nquads);
+        GBSW_HIP_TRY(ex.bnd.reserve((size_t)std::max(gf, gr) * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t)));
+        rc = launch_timed(ex, st0, kPlainForward, K, gf, lds, a, ex.bnd.ptr);
+        if (rc) return rc;
+        ex.launches.back().pairs = job.n;
+        ex.launches.back().cells = job.cells;
+        rc = launch_timed(ex, st0, kPlainReverse, K, gr, lds, a, ex.bnd.ptr);
+        if (rc) return rc;
+        ex.launches.back().pairs = job.n;
+    } else {
+        // device binning: keys, then a stable radix sort into the work order
+        GBSW_HIP_TRY(bin_keys(job.ref_off, job.qry_off, job.n, job.bucket_of_len, job.keys, job.vals, st0));
+        size_t tb = job.sort_temp_bytes;
+        GBSW_HIP_TRY(sort_pairs(job.sort_temp, tb, job.keys, job.keys_sorted, job.vals, job.order, (int)job.n, 20, st0));
+        const size_t nb = job.buckets.size();
+        if (ex.queue.bytes < sizeof(uint32_t) * nb) GBSW_HIP_TRY(ex.queue.reserve(sizeof(uint32_t) * nb));
+        GBSW_HIP_TRY(zero_u32(ex.queue.as<uint32_t>(), (int)nb, st0));
+        // Buckets run concurrently on 1 + kAuxStreams streams, the ones whose single
+        // work item takes longest (largest K) first, so a bucket with few long
+        // pairs overlaps the bulk instead of running alone at the end.
+        std::vector<size_t> order(nb);
+        std::vector<int> grids(nb), ldss(nb);
+        size_t bnd_words = 0;
+        for (size_t i = 0; i < nb; ++i) {
+            order[i] = i;
+            const Bucket& bk = job.buckets[i];
+            ldss[i] = lds_for(bk.K, job.alpha);
+            grids[i] = grid_for(ex, kBatched, bk.K, ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
+            bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+        }
+        std::sort(order.begin(), order.end(),
+                  [&](size_t x, size_t y) { return job.buckets[x].K > job.buckets[y].K; });
+        GBSW_HIP_TRY(ex.bnd.reserve(std::max<size_t>(bnd_words, 1) * sizeof(uint32_t)));
+        const int ns = bucket_streams();
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_fork, st0));
+        for (int i = 0; i < ns - 1; ++i) GBSW_HIP_TRY(hipStreamWaitEvent(ex.aux[i], ex.ev_fork, 0));
+        size_t bnd_at = 0;
+        for (size_t j = 0; j < order.size(); ++j) {
+            const size_t i = order[j];
+            const Bucket& bk = job.buckets[i];
+            a.order = job.order + bk.begin;
+            a.order_n = bk.count;
+            a.queue = ex.queue.as<uint32_t>() + i;
+            hipStream_t st = (j % ns) == 0 ? st0 : ex.aux[j % ns - 1];
+            rc = launch_timed(ex, st, kBatched, bk.K, grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
+            if (rc) return rc;
+            ex.launches.back().pairs = bk.count;
+            ex.launches.back().cells = bk.cells;
+            bnd_at += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+        }
+        for (int i = 0; i < ns - 1; ++i) {
+            GBSW_HIP_TRY(hipEventRecord(ex.ev_join[i], ex.aux[i]));
+            GBSW_HIP_TRY(hipStreamWaitEvent(st0, ex.ev_join[i], 0));
+        }
+    }
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.status_host.ptr, ex.status.ptr, sizeof(int32_t) * kStatusWords,
+                                hipMemcpyDeviceToHost, st0));
+    GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st0));
+    return GBSW_OK;
+}
+
+int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* device_ms)
+{
+    GBSW_HIP_TRY(hipEventSynchronize(ex.ev_stop));
+    int32_t* status = ex.status_host.as<int32_t>();
+    if (status[kStatusSaturated] > 0) {
+        // pairs whose int16 score reached the saturation limit: recompute them
+        // (forward + reverse) with int32 cells
+        int nk = 0;
+        const int* wk = wide_k_list(&nk);
+        int K = wk[nk - 1];
+        for (int i = 0; i < nk; ++i)
+            if (kVirtPerPair * wk[i] >= job.max_cols) { K = wk[i]; break; }
+        const int lds = lds_for(K, job.alpha, 8);
+        const int64_t count = status[kStatusSaturated];
+        KernelArgs a = base_args(ex, job, sc);
+        a.order = ex.redo.as<int32_t>();
+        a.order_n = count;
+        a.sat_limit = INT_MAX;
+        a.queue = ex.queue.as<uint32_t>();
+        GBSW_HIP_TRY(zero_u32(a.queue, 1, ex.stream));
+        const int grid = grid_for(ex, kBatchedWide, K, lds, (count + kPairsPerWave - 1) / kPairsPerWave);
+        GBSW_HIP_TRY(ex.bnd.reserve((size_t)grid * kPairsPerWave * (size_t)a.bnd_rows * 8));
+        int rc = launch_timed(ex, ex.stream, kBatchedWide, K, grid, lds, a, ex.bnd.ptr);
+        if (rc) return rc;
+        ex.launches.back().pairs = count;
+        GBSW_HIP_TRY(hipMemcpyAsync(ex.status_host.ptr, ex.status.ptr, sizeof(int32_t) * kStatusWords,
+                                    hipMemcpyDeviceToHost, ex.stream));
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, ex.stream));
+        GBSW_HIP_TRY(hipEventSynchronize(ex.ev_stop));
+    }
+    float ms = 0.f;
+    GBSW_HIP_TRY(hipEventElapsedTime(&ms, ex.ev_start, ex.ev_stop));
+    if (device_ms) *device_ms = ms;
+    for (size_t i = 0; i < ex.launches.size(); ++i) {
+        float t = 0.f;
+        GBSW_HIP_TRY(hipEventElapsedTime(&t, ex.kev[2 * i], ex.kev[2 * i + 1]));
+        ex.launches[i].ms = t;
+    }
+    if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
+    return GBSW_OK;
+}
+
+} // namespace gbsw

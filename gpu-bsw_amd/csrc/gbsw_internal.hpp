@@ -1,0 +1,132 @@
+// gbsw_internal.hpp -- host-side building blocks shared by the resident-batch
+// API (gbsw_api.cpp) and the serving loop (gbsw_server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gbsw.h"
+#include "gbsw_kernels.hpp"
+
+namespace gbsw {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define GBSW_HIP_TRY(expr)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::gbsw::fail(GBSW_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release();
+    hipError_t reserve(size_t b);
+    template <class T> T* as() const { return (T*)ptr; }
+};
+
+struct PinnedBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() { release(); }
+    void release();
+    hipError_t reserve(size_t b);
+    template <class T> T* as() const { return (T*)ptr; }
+};
+
+// One length bucket of the batched form: a kernel geometry and a slice of the
+// sorted work order.
+struct Bucket {
+    int K = 0;
+    int64_t begin = 0, count = 0;
+    int64_t cells = 0;
+};
+
+struct LaunchRec {
+    int kind, K, grid;
+    int64_t pairs, cells;
+    double ms;
+};
+
+constexpr int kAuxStreams = 3;
+
+// Streams, events and scratch of one in-flight alignment job.
+struct Exec {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t aux[kAuxStreams] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[kAuxStreams] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::vector<hipEvent_t> kev; // per-launch event pairs
+    DeviceBuffer bnd, queue, status, sub, lut, colmax, redo;
+    PinnedBuffer status_host;    // status words copied back at the end of a job
+    PinnedBuffer tables_host;    // staging of sub / lut / colmax for the async copies
+    std::vector<LaunchRec> launches;
+    int32_t max_sub = 0;
+    int init(int device, int num_cu);
+    void destroy();
+};
+
+// Device view of the pairs of one job (a resident batch, or a serving slot).
+struct Job {
+    int32_t alphabet = GBSW_DNA;
+    int64_t n = 0;
+    const uint8_t* ref = nullptr;
+    const uint8_t* qry = nullptr;
+    const uint64_t* ref_off = nullptr;
+    const uint64_t* qry_off = nullptr;
+    int32_t* out = nullptr;          // 5*n int32
+    int32_t alpha = 0;               // residue codes
+    const uint8_t* lut_host = nullptr; // 256 entries
+    int32_t max_rows = 0, max_cols = 0;
+    int64_t cells = 0;
+    std::vector<Bucket> buckets;     // batched form (host counts)
+    const uint8_t* bucket_of_len = nullptr; // device table for the keys kernel
+    uint32_t* keys = nullptr;
+    uint32_t* keys_sorted = nullptr;
+    int32_t* vals = nullptr;
+    int32_t* order = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+};
+
+// Largest compiled K that fits a 64 KiB workgroup for this alphabet size.
+int max_k_for(int alpha);
+// K for a shorter sequence of `cols` residues (strips x (K + 2) minimised).
+int k_for_cols(int cols, int kmax);
+// Host table shorter-length -> bucket index (position of its K in kernel_k_list),
+// 65536 entries, cached per kmax.
+const std::vector<uint8_t>& bucket_table(int kmax);
+// Fill job.buckets from per-pair lengths (host), ordered by ascending K.
+void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry);
+// Workgroups of a full-occupancy batched launch for K columns per lane (the grid
+// of a bucket is min(this, ceil(pairs / kPairsPerWave))).
+int batched_grid_cap(const Exec& ex, int K, int alpha);
+// Scratch bytes of the radix sort for n pairs.
+int sort_scratch_bytes(int64_t n, size_t* bytes);
+
+// Validate a scoring for a job and build its substitution table.
+int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub);
+
+// Enqueue a whole alignment job (binning if batched, kernels) on ex.stream.
+// Asynchronous: ex.ev_stop is recorded after the last kernel and the status words
+// are copied to ex.status_host.
+int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode);
+// After ex.ev_stop completed: recompute int16-saturated pairs with the int32
+// kernel (synchronously, rare), check the status words and collect timings.
+int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* device_ms);
+
+} // namespace gbsw

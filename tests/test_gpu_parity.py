@@ -234,13 +234,15 @@ def test_dropin_cpp_caller(tmp_path):
     import subprocess
     from test_abi import build_dropin_caller
     exe = build_dropin_caller(tmp_path)
-    out = subprocess.run([exe, "dna"], capture_output=True, text=True, check=True).stdout.splitlines()
+    full = subprocess.run([exe, "dna"], capture_output=True, text=True, check=True).stdout.splitlines()
+    # the driver's own report lines (driver.cpp:240-242) come first; rows are tab-separated
+    out = [l for l in full if "\t" in l]
+    assert any(l.startswith("Total Execution Time (seconds):") for l in full)
     assert out[0] == "4\t5\t8\t5\t8"          # score, ref_begin, ref_end-1, query_begin, query_end-1
     assert out[1] == "4\t0\t3\t0\t3"
     sc = ol.dna_scoring(1, -3, -3, -1)
     s, rb, re_, qb, qe = ol.align("TTTTACGTACGTTTTT", "ACGTACGT", sc)
     assert out[2] == "%d\t%d\t%d\t%d\t%d" % (s, rb, re_ - 1, qb, qe - 1)
-    assert any(l.startswith("Total Execution Time (seconds):") for l in
-               subprocess.run([exe, "dna"], capture_output=True, text=True).stdout.splitlines())
-    outa = subprocess.run([exe, "aa"], capture_output=True, text=True, check=True).stdout.splitlines()
+    outa = [l for l in subprocess.run([exe, "aa"], capture_output=True, text=True, check=True).stdout.splitlines()
+            if "\t" in l]
     assert outa[1].split("\t")[0] == "33"      # WWW vs AWWWA: 3 x 11
