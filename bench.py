@@ -9,6 +9,14 @@ counted, forward + reverse timed).  The same batch is also timed in the plain
 form (one forward and one reverse launch, input order, one geometry), and the
 CPU oracle is timed on a bounded sample of the same workload.
 
+Serving (the "p99 latency under load" half of the metric): a server
+(gbsw_server_*) is driven by an open-loop Poisson generator over a pool of
+requests of 1..4096 pairs (70% DNA like the mix above, 30% protein); its peak is
+measured by submitting the whole pool at once, then each offered load (25, 50,
+75, 100% of that peak) runs for --serving-window seconds and reports p50/p99
+request latency and sustained GCUPS.  Every completed request is compared with
+the same request aligned alone.
+
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -72,6 +80,49 @@ def time_mode(batch, scoring, mode, warmup, steps, cells):
             "launches": avg, "buckets": st["buckets"]}
 
 
+def serving_bench(ctx, args):
+    from gbsw import loadgen
+    t = time.perf_counter()
+    pool = loadgen.Pool(args.serving_requests)
+    log("serving pool: %d requests, %d pairs, mean %.3g cells/request, built in %.1f s" %
+        (len(pool.requests), sum(r["n"] for r in pool.requests), pool.mean_cells, time.perf_counter() - t))
+    # each request aligned alone: the results every served copy must reproduce
+    alone = []
+    for r in pool.requests:
+        w = r["workload"]
+        res = ctx.align_packed(w.ref, w.ref_off, w.query, w.query_off, pool.scorings[w.alphabet], "batched")
+        alone.append(np.stack([res[f] for f in gbsw.FIELDS]))
+    mismatches = [0]
+
+    def verify(k, out, info):
+        n = alone[k].shape[1]
+        if info["status"] != 0 or not np.array_equal(out[:, :n], alone[k]):
+            mismatches[0] += 1
+
+    srv = gbsw.Server(0, slots=args.serving_slots)
+    loadgen.saturate(srv, pool, repeat=1)  # warm-up
+    peak, infos = loadgen.saturate(srv, pool, repeat=2)
+    log("serving peak (whole pool x2 at once): %.1f GCUPS" % peak)
+    loads = []
+    for frac in [float(x) for x in args.serving_loads.split(",")]:
+        rate = frac * peak * 1e9 / pool.mean_cells
+        r = loadgen.poisson(srv, pool, rate, args.serving_window, seed=int(frac * 1000), verify=verify)
+        log("  load %.0f%%: offered %.1f GCUPS (%.0f req/s), sustained %.1f GCUPS, p50 %.2f ms, p99 %.2f ms, "
+            "%d requests, submit lag p99 %.2f ms" % (100 * frac, r["offered_gcups"], rate, r["gcups"], r["p50_ms"],
+                                                     r["p99_ms"], r["requests"], r["lag_p99_ms"]))
+        loads.append({"load": frac, "offered_gcups": round(r["offered_gcups"], 1), "gcups": round(r["gcups"], 1),
+                      "req_per_s": round(rate, 1), "requests": r["requests"], "p50_ms": round(r["p50_ms"], 3),
+                      "p99_ms": round(r["p99_ms"], 3), "submit_lag_p99_ms": round(r["lag_p99_ms"], 3),
+                      "errors": r["errors"]})
+    st = srv.stats()
+    srv.close()
+    pool.close()
+    return {"peak_gcups": round(peak, 1), "window_s": args.serving_window, "slots": args.serving_slots,
+            "pool_requests": len(pool.requests), "loads": loads, "micro_batches": st["micro_batches"],
+            "requests": st["completed"], "hbm_bytes": st["device_bytes"],
+            "bit_exact_vs_alone": mismatches[0] == 0, "mismatches": mismatches[0]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +134,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-window", type=float, default=5.0, help="seconds per offered load")
+    ap.add_argument("--serving-loads", default="0.25,0.5,0.75,1.0")
+    ap.add_argument("--serving-requests", type=int, default=512, help="request pool size")
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (rocprof FETCH/WRITE calibration)")
     args = ap.parse_args()
@@ -140,6 +196,13 @@ def main():
                 "kernel_ms": round(dom["ms"], 3), "ops_per_cell": OPS_PER_CELL,
                 "aggregate_achieved": round(agg, 3), "aggregate_frac": round(agg / PEAK_INT_TOPS_PACKED, 4)}
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench(ctx, args)
+        if not serving["bit_exact_vs_alone"]:
+            log("ERROR: %d served requests differ from the same requests aligned alone" % serving["mismatches"])
+            sys.exit(1)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_pairs, args.cpu_threads)
@@ -169,6 +232,7 @@ def main():
                                               "ms_per_step": round(plain["ms_per_step"], 3)},
         "batched_over_plain": None if plain is None else round(batched["gcups"] / plain["gcups"], 3),
         "bit_exact_plain_vs_batched": plain is not None,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     batch.close()

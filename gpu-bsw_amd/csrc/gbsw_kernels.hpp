@@ -21,4 +21,16 @@ hipError_t bin_keys(const uint64_t* ref_off, const uint64_t* qry_off, int64_t n,
 hipError_t sort_pairs(void* temp, size_t& temp_bytes, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
                       int32_t* vout, int n, int end_bit, hipStream_t stream);
 
+// gbsw_gather.hip: one tile = 1..kGatherTileBytes bytes of one request part,
+// copied from src to dst; part_flags[part] |= 1 when a byte has accept[byte] == 0.
+constexpr uint32_t kGatherTileBytes = 32768;
+struct GatherTile {
+    const uint8_t* src; // device-readable pinned host memory
+    uint8_t* dst;       // device memory
+    uint32_t len;
+    int32_t part;
+};
+hipError_t gather(const GatherTile* tiles, int ntiles, const uint8_t* accept, int32_t* part_flags,
+                  hipStream_t stream);
+
 } // namespace gbsw

@@ -24,7 +24,9 @@ MODES = {"plain": PLAIN, "batched": BATCHED}
 FIELDS = ("score", "ref_begin", "ref_end", "query_begin", "query_end")
 
 GBSW_OK = 0
-ERRORS = {-1: "ARGUMENT", -2: "RESIDUE", -3: "SCORING", -4: "DEVICE", -5: "MEMORY", -6: "RANGE", -7: "INTERNAL"}
+ERRORS = {-1: "ARGUMENT", -2: "RESIDUE", -3: "SCORING", -4: "DEVICE", -5: "MEMORY", -6: "RANGE", -7: "INTERNAL",
+          -8: "TIMEOUT"}
+E_RESIDUE, E_TIMEOUT = -2, -8
 
 
 class GbswError(RuntimeError):
@@ -65,11 +67,49 @@ class LaunchStat(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_int32),
+        ("pairs_per_slot", ctypes.c_int32),
+        ("residues_per_slot", ctypes.c_int64),
+        ("boundary_bytes_per_slot", ctypes.c_int64),
+        ("staging_bytes", ctypes.c_int64),
+        ("dna_alphabet", ctypes.c_char * 32),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("ticket", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("parts", ctypes.c_int32),
+        ("pairs", ctypes.c_int64),
+        ("cells", ctypes.c_int64),
+        ("queue_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+    ]
+
+
+class ServingStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_int64),
+        ("completed", ctypes.c_int64),
+        ("pending", ctypes.c_int64),
+        ("micro_batches", ctypes.c_int64),
+        ("pairs", ctypes.c_int64),
+        ("cells", ctypes.c_int64),
+        ("busy_ms", ctypes.c_double),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
 # Every symbol include/gbsw.h declares (checked by the CPU test suite).
 EXPORTED = (
     "gbsw_context_create", "gbsw_context_destroy", "gbsw_last_error", "gbsw_batch_create",
     "gbsw_batch_destroy", "gbsw_batch_align", "gbsw_batch_results", "gbsw_batch_device_results",
     "gbsw_batch_stats", "gbsw_batch_launch_stats", "gbsw_align", "gbsw_device_memory", "gbsw_version",
+    "gbsw_server_config_default", "gbsw_server_create", "gbsw_server_destroy", "gbsw_server_submit",
+    "gbsw_server_wait", "gbsw_server_poll", "gbsw_server_stats", "gbsw_host_alloc", "gbsw_host_free",
 )
 
 _lib = None
@@ -102,8 +142,20 @@ def load_library():
     lib.gbsw_batch_launch_stats.argtypes = [vp, i32, ctypes.POINTER(LaunchStat)]
     lib.gbsw_align.argtypes = [vp, ctypes.POINTER(Scoring), i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.gbsw_device_memory.argtypes = [i32, ctypes.POINTER(ctypes.c_uint64)]
+    u64 = ctypes.c_uint64
+    lib.gbsw_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
+    lib.gbsw_server_config_default.restype = None
+    lib.gbsw_server_create.argtypes = [i32, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    lib.gbsw_server_destroy.argtypes = [vp]
+    lib.gbsw_server_submit.argtypes = [vp, ctypes.POINTER(Scoring), i64, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       ctypes.POINTER(u64)]
+    lib.gbsw_server_wait.argtypes = [vp, u64, ctypes.c_double, ctypes.POINTER(RequestInfo)]
+    lib.gbsw_server_poll.argtypes = [vp, i32, ctypes.POINTER(RequestInfo), ctypes.POINTER(i32)]
+    lib.gbsw_server_stats.argtypes = [vp, ctypes.POINTER(ServingStats)]
+    lib.gbsw_host_alloc.argtypes = [u64, ctypes.POINTER(vp)]
+    lib.gbsw_host_free.argtypes = [vp]
     for name in EXPORTED:
-        if name not in ("gbsw_last_error", "gbsw_version"):
+        if name not in ("gbsw_last_error", "gbsw_version", "gbsw_server_config_default"):
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -258,3 +310,112 @@ def kernel_driver_aa(reads, contigs, scoring_matrix, open_gap, extend_gap, facto
     n = len(contigs)
     sc = protein_scoring(scoring_matrix, int(open_gap), int(extend_gap))
     return default_context().align(list(contigs), list(reads)[:n], sc, mode)
+
+
+# ---------------------------------------------------------------------------
+# Serving (gbsw_server_*): asynchronous requests with continuous batching.
+
+class HostBuffer:
+    """Pinned host memory from gbsw_host_alloc, viewed as a numpy uint8 array.
+    Requests whose sequences lie in such a buffer are read by the device in place
+    (no staging copy); the buffer must then stay unchanged until they complete."""
+
+    def __init__(self, nbytes):
+        self._lib = load_library()
+        self._p = ctypes.c_void_p()
+        _check(self._lib.gbsw_host_alloc(max(int(nbytes), 1), ctypes.byref(self._p)))
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * max(int(nbytes), 1)).from_address(self._p.value))
+
+    def close(self):
+        if self._p:
+            self.array = None
+            self._lib.gbsw_host_free(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def server_config(**kw):
+    cfg = ServerConfig()
+    load_library().gbsw_server_config_default(ctypes.byref(cfg))
+    for k, v in kw.items():
+        if k == "dna_alphabet":
+            v = v.encode() if isinstance(v, str) else bytes(v)
+        setattr(cfg, k, v)
+    return cfg
+
+
+def _info(ri):
+    return {k: getattr(ri, k) for k, _ in RequestInfo._fields_}
+
+
+class Server:
+    """gbsw_server: submit() returns a ticket at once; wait()/poll() report
+    completed requests.  Arrays handed to submit() are kept alive here until the
+    request is reported."""
+
+    def __init__(self, device=0, **config):
+        self._lib = load_library()
+        self._h = ctypes.c_void_p()
+        self._keep = {}
+        cfg = server_config(**config)
+        _check(self._lib.gbsw_server_create(device, ctypes.byref(cfg), ctypes.byref(self._h)))
+
+    def submit(self, scoring, ref, ref_off, query, query_off, out=None):
+        """Queue the pairs; `out` (5 x n int32, optional) receives the results.
+        Returns (ticket, out)."""
+        n = len(ref_off) - 1
+        if len(query_off) - 1 != n:
+            raise ValueError("ref and query counts differ")
+        ref_off = np.ascontiguousarray(ref_off, dtype=np.uint64)
+        query_off = np.ascontiguousarray(query_off, dtype=np.uint64)
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        query = np.ascontiguousarray(query, dtype=np.uint8)
+        if out is None:
+            out = np.zeros((5, max(n, 1)), dtype=np.int32)
+        assert out.dtype == np.int32 and out.flags["C_CONTIGUOUS"] and out.shape[0] == 5 and out.shape[1] >= n
+        t = ctypes.c_uint64()
+        base = out.ctypes.data
+        stride = out.strides[0]
+        _check(self._lib.gbsw_server_submit(self._h, ctypes.byref(scoring), n, ref.ctypes.data, ref_off.ctypes.data,
+                                            query.ctypes.data, query_off.ctypes.data, base, base + stride,
+                                            base + 2 * stride, base + 3 * stride, base + 4 * stride,
+                                            ctypes.byref(t)))
+        self._keep[t.value] = (ref, query, out)
+        return t.value, out
+
+    def wait(self, ticket, timeout=-1.0):
+        ri = RequestInfo()
+        _check(self._lib.gbsw_server_wait(self._h, ticket, float(timeout), ctypes.byref(ri)))
+        self._keep.pop(ticket, None)
+        return _info(ri)
+
+    def poll(self, max_count=1024):
+        infos = (RequestInfo * max_count)()
+        cnt = ctypes.c_int32()
+        _check(self._lib.gbsw_server_poll(self._h, max_count, infos, ctypes.byref(cnt)))
+        res = [_info(infos[i]) for i in range(cnt.value)]
+        for r in res:
+            self._keep.pop(r["ticket"], None)
+        return res
+
+    def stats(self):
+        st = ServingStats()
+        _check(self._lib.gbsw_server_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServingStats._fields_}
+
+    def close(self):
+        if self._h:
+            self._lib.gbsw_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

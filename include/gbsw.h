@@ -119,6 +119,78 @@ int gbsw_align(gbsw_context* ctx, const gbsw_scoring* scoring, int32_t mode, int
 /* Total device memory in bytes (driver.cpp:10-14, get_tot_gpu_mem). */
 int gbsw_device_memory(int32_t device, uint64_t* bytes);
 
+/* ------------------------------------------------------------------------
+ * Serving: an asynchronous request loop with continuous batching.
+ *
+ * No counterpart in the reference, whose driver is one blocking call per batch
+ * (driver.hpp:32-37).  A server owns `slots` micro-batch slots, each a set of
+ * device-resident buffers (sequences, offsets, results, binning and boundary
+ * scratch) of fixed capacity, with its own streams.  A scheduler thread packs
+ * pending requests (oldest first, same scoring) into a free slot as soon as one
+ * frees up, so requests that arrive while others run are admitted and overlap
+ * on the device; admission is bounded by each slot's HBM footprint (pairs,
+ * residue bytes, strip-boundary bytes) and large requests are split across
+ * micro-batches.  Results are bit-identical to gbsw_align on the same pairs,
+ * whatever the mix they were batched with.
+ * ------------------------------------------------------------------------ */
+#define GBSW_E_TIMEOUT (-8)   /* gbsw_server_wait: the request is not complete yet */
+
+typedef struct gbsw_server gbsw_server;
+
+typedef struct {
+    int32_t slots;                   /* micro-batches in flight (default 4)                 */
+    int32_t pairs_per_slot;          /* pair capacity of one micro-batch (default 262144)   */
+    int64_t residues_per_slot;       /* ref+query bytes of one micro-batch (default 256 MiB) */
+    int64_t boundary_bytes_per_slot; /* strip-boundary scratch of one micro-batch (default 1 GiB) */
+    int64_t staging_bytes;           /* pinned host arena for copied requests (default 256 MiB) */
+    char dna_alphabet[32];           /* DNA bytes accepted, NUL-terminated (default "ACGTN"); other
+                                        bytes fail the request with GBSW_E_RESIDUE          */
+} gbsw_server_config;
+
+typedef struct {
+    uint64_t ticket;
+    int32_t status;       /* GBSW_OK or the request's error code */
+    int32_t parts;        /* micro-batches the request was split over */
+    int64_t pairs;
+    int64_t cells;
+    double queue_ms;      /* submit -> first micro-batch launched */
+    double latency_ms;    /* submit -> results written */
+} gbsw_request_info;
+
+typedef struct {
+    int64_t submitted, completed, pending, micro_batches;
+    int64_t pairs, cells;            /* of completed requests                     */
+    double busy_ms;                  /* device time summed over micro-batches     */
+    int64_t device_bytes;            /* HBM held by the server                    */
+} gbsw_serving_stats;
+
+void gbsw_server_config_default(gbsw_server_config* cfg);
+int gbsw_server_create(int32_t device, const gbsw_server_config* cfg, gbsw_server** out);
+/* Completes every submitted request, then stops the scheduler and frees the server. */
+int gbsw_server_destroy(gbsw_server* srv);
+
+/* Queue n pairs (layout as gbsw_batch_create) for alignment under `scoring`.
+ * Returns at once with a ticket; the five n-entry int32 result arrays are
+ * written before the request is reported complete and must stay valid until
+ * then.  Input buffers allocated with gbsw_host_alloc are read by the device in
+ * place and must stay unchanged until completion; other input buffers are
+ * copied into the server's pinned arena and may be reused when this returns.
+ * Thread-safe. */
+int gbsw_server_submit(gbsw_server* srv, const gbsw_scoring* scoring, int64_t n, const char* ref,
+                       const uint64_t* ref_off, const char* query, const uint64_t* query_off,
+                       int32_t* score, int32_t* ref_begin, int32_t* ref_end, int32_t* query_begin,
+                       int32_t* query_end, uint64_t* ticket);
+/* Block until the request completes (timeout_s < 0: no limit) and report it.
+ * A reported ticket is released and unknown afterwards. */
+int gbsw_server_wait(gbsw_server* srv, uint64_t ticket, double timeout_s, gbsw_request_info* info);
+/* Report up to `max` completed requests not reported yet, in completion order. */
+int gbsw_server_poll(gbsw_server* srv, int32_t max, gbsw_request_info* infos, int32_t* count);
+int gbsw_server_stats(gbsw_server* srv, gbsw_serving_stats* out);
+
+/* Pinned host memory the device reads in place (zero-copy submission). */
+int gbsw_host_alloc(uint64_t bytes, void** out);
+int gbsw_host_free(void* p);
+
 /* Version string of the library. */
 const char* gbsw_version(void);
 
