@@ -49,10 +49,11 @@ __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b)
     asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
     return r;
 }
-// Rotate right by one lane inside each 16-lane row (DPP row_ror:1).
+// Rotate right by one lane inside each 16-lane row (DPP row_ror:1).  Every lane
+// reads a valid source, so the destination needs no initial value.
 __device__ __forceinline__ uint32_t row_ror1(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x121, 0xf, 0xf, false);
 }
 __device__ __forceinline__ int32_t lo16(uint32_t v) { return (int32_t)(int16_t)(v & 0xFFFFu); }
 __device__ __forceinline__ int32_t hi16(uint32_t v) { return (int32_t)(int16_t)(v >> 16); }

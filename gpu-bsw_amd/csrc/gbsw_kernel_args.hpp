@@ -10,6 +10,10 @@ constexpr int kPairsPerWave = 4;
 constexpr int kWave = 64;
 constexpr int kRing = 128;   // row-stream ring entries per pair
 constexpr int kChunk = 32;   // rows refilled at a time
+// The row-code ring keeps a copy of its first kChunk entries after its end, so
+// the 32 consecutive entries a virtual lane reads during one chunk are
+// contiguous whatever the chunk's position in the ring.
+constexpr int kRingCodes = kRing + kChunk;
 constexpr int kMaxAlpha = 31; // codes 0..A-1, row A is the padding row
 constexpr int16_t kNeg = -16384;
 
@@ -44,7 +48,9 @@ struct KernelArgs {
 };
 
 // Dynamic LDS layout of one 64-lane workgroup; W = profile columns per row
-// (32 virtual lanes x kstride(K)).
+// (32 virtual lanes x kstride(K)).  The candidate exchange of a pass's final
+// reduction runs after every group has left its sweep, so it reuses the start
+// of the group areas.
 struct LdsLayout {
     int32_t lut_off, sub_off, colmax_off, xch_off, info_off, group_off, group_bytes;
     int32_t prof_bytes, ring_code_off, ring_bnd_off, total;
@@ -67,14 +73,14 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha, int32_
 {
     LdsLayout L;
     L.lut_off = 0;                                       // 256 B
-    L.sub_off = 256;                                     // kMaxAlpha^2 int16
-    L.colmax_off = L.sub_off + 2 * 32 * 32;              // 32 int16
-    L.xch_off = align_up(L.colmax_off + 2 * 32, 16);     // 64 lanes x 2 keys x 8 B
-    L.info_off = L.xch_off + 64 * 2 * 8;                 // 4 groups x 8 int32
+    L.sub_off = 256;                                     // alpha^2 int16
+    L.colmax_off = L.sub_off + align_up(2 * alpha * alpha, 16); // 32 int16
+    L.info_off = L.colmax_off + 2 * 32;                  // 4 groups x 8 int32
     L.group_off = align_up(L.info_off + 4 * 8 * 4, 128);
+    L.xch_off = L.group_off;                             // 64 lanes x 2 keys x 8 B
     L.prof_bytes = align_up((alpha + 1) * W * 2, 128);
     L.ring_code_off = L.prof_bytes;
-    L.ring_bnd_off = L.ring_code_off + kRing * 2;
+    L.ring_bnd_off = L.ring_code_off + kRingCodes * 2;
     L.group_bytes = align_up(L.ring_bnd_off + kRing * bnd_bytes, 128);
     L.total = L.group_off + kPairsPerWave * L.group_bytes;
     return L;

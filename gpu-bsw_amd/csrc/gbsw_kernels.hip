@@ -182,8 +182,13 @@ __device__ __forceinline__ void row_store(const RowFetch<A>& f, int chunk, int l
     int c1 = f.b1 < 0 ? alpha : (int)lut[f.b1];
     c0 = c0 > alpha ? alpha : c0; // bytes outside the alphabet read the padding row
     c1 = c1 > alpha ? alpha : c1;
-    ring_code[x0 & (kRing - 1)] = (uint16_t)(c0 * rowstride);
-    ring_code[x1 & (kRing - 1)] = (uint16_t)(c1 * rowstride);
+    const int i0 = x0 & (kRing - 1), i1 = x1 & (kRing - 1);
+    ring_code[i0] = (uint16_t)(c0 * rowstride);
+    ring_code[i1] = (uint16_t)(c1 * rowstride);
+    if (i0 < kChunk) { // the copy after the ring's end
+        ring_code[i0 + kRing] = (uint16_t)(c0 * rowstride);
+        ring_code[i1 + kRing] = (uint16_t)(c1 * rowstride);
+    }
     ring_bnd[x0 & (kRing - 1)] = f.d0;
     ring_bnd[x1 & (kRing - 1)] = f.d1;
 }
@@ -350,15 +355,13 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             // so the compiler will not unroll a runtime trip count); a step past
             // steps_max only meets padding rows
             const int s_end = min(s0 + kChunk, steps_max);
-            auto step = [&](const int s) {
+            auto step = [&](const int s, const uint32_t off_lo, const uint32_t off_hi) {
                 // left inputs: neighbour's outputs of the previous step; virtual lane 0
                 // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
                 V xH = A::dpp(Hout), xE = A::dpp(Eout);
-                const Bd bnd = multi_strip ? ring_bnd[s & (kRing - 1)] : A::zero_bnd();
+                const Bd bnd = ring_bnd[s & (kRing - 1)]; // zero unless a strip precedes
                 xH = A::fix_h(xH, bnd, lane0_mask);
                 xE = A::fix_e(xE, bnd, lane0_mask);
-                const uint32_t off_lo = ring_code[(s - v_lo) & (kRing - 1)];
-                const uint32_t off_hi = ring_code[(s - v_hi) & (kRing - 1)];
                 uint32_t plo[KS / 2], phi[KS / 2];
                 prof_load<K>((const uint8_t*)prof + off_lo, v_lo, plo);
                 prof_load<K>((const uint8_t*)prof + off_hi, v_hi, phi);
@@ -415,9 +418,14 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 }
                 rowpk = A::rows_next(rowpk);
             };
+            // this chunk's row codes of each virtual lane are contiguous in the ring
+            const uint16_t* rc_lo = ring_code + ((s0 - v_lo) & (kRing - 1));
+            const uint16_t* rc_hi = ring_code + ((s0 - v_hi) & (kRing - 1));
             for (int s = s0; s < s_end; s += 2) {
-                step(s);
-                step(s + 1);
+                step(s, rc_lo[0], rc_hi[0]);
+                step(s + 1, rc_lo[1], rc_hi[1]);
+                rc_lo += 2;
+                rc_hi += 2;
             }
         }
 
