@@ -49,6 +49,13 @@ __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b)
     asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
     return r;
 }
+// v_bfi_b32 with a wave-uniform (SGPR) second operand.
+__device__ __forceinline__ uint32_t bfi_s(uint32_t m, uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "s"(a), "v"(b));
+    return r;
+}
 // Rotate right by one lane inside each 16-lane row (DPP row_ror:1).  Every lane
 // reads a valid source, so the destination needs no initial value.
 __device__ __forceinline__ uint32_t row_ror1(uint32_t v)

@@ -64,6 +64,23 @@ __device__ __forceinline__ uint32_t inv_prio(uint32_t p)
 
 // ---------------------------------------------------------------- arithmetic
 
+// Per-lane constants of the lane-0 fix-up after the row rotation: in lane 0 the
+// low virtual lane (0) takes the strip boundary and the high one (16) takes the
+// rotated low half (virtual lane 15); other lanes keep the rotated value.
+struct LaneFix {
+    uint32_t mask; // all ones in lane 0
+    uint32_t selh; // v_perm selector: lane 0 -> (boundary H, rotated low), else identity
+    uint32_t sele; // the same with the boundary's E half
+};
+__device__ __forceinline__ LaneFix lane_fix(int l)
+{
+    LaneFix f;
+    f.mask = l == 0 ? 0xFFFFFFFFu : 0u;
+    f.selh = l == 0 ? 0x05040100u : 0x07060504u;
+    f.sele = l == 0 ? 0x05040302u : 0x07060504u;
+    return f;
+}
+
 // Packed16: one VGPR holds the cells of both virtual lanes of a lane as int16
 // halves (low: virtual lane l, high: l+16); every op advances two cells.
 struct Packed16 {
@@ -72,14 +89,14 @@ struct Packed16 {
     static constexpr bool kWide = false;
     __device__ static V zero() { return 0u; }
     __device__ static V splat(int32_t x) { return pack16(x, x); }
-    __device__ static V rows_init(int v_lo, int v_hi) { return pack16(-v_lo, -v_hi); }
-    __device__ static V rows_next(V r) { return raw(as_u(r) + (u16x2){1, 1}); }
-    __device__ static int32_t row_lo(V r) { return (int32_t)(r & 0xFFFFu); }
-    __device__ static int32_t row_hi(V r) { return (int32_t)(r >> 16); }
+    // step numbers recorded in both halves (step < 2^16)
+    __device__ static V sel_step(V m, int s, V b) { return bfi_s(m, pack16(s, s), b); }
+    __device__ static int32_t step_lo(V r) { return (int32_t)(r & 0xFFFFu); }
+    __device__ static int32_t step_hi(V r) { return (int32_t)(r >> 16); }
     __device__ static V dpp(V x) { return row_ror1(x); }
     // virtual lane 0 takes the strip boundary, virtual lane 16 the low half of lane 15
-    __device__ static V fix_h(V x, B bnd, uint32_t m) { return bfi(m, __builtin_amdgcn_perm(x, bnd, 0x05040100u), x); }
-    __device__ static V fix_e(V x, B bnd, uint32_t m) { return bfi(m, __builtin_amdgcn_perm(x, bnd, 0x05040302u), x); }
+    __device__ static V fix_h(V x, B bnd, const LaneFix& f) { return __builtin_amdgcn_perm(x, bnd, f.selh); }
+    __device__ static V fix_e(V x, B bnd, const LaneFix& f) { return __builtin_amdgcn_perm(x, bnd, f.sele); }
     __device__ static B bnd_out(V h, V e) { return __builtin_amdgcn_perm(e, h, 0x07060302u); } // high halves
     template <int K>
     __device__ static V prof(const uint32_t* plo, const uint32_t* phi, int k)
@@ -108,18 +125,17 @@ struct Wide32 {
     static constexpr bool kWide = true;
     __device__ static V zero() { return V{0, 0}; }
     __device__ static V splat(int32_t x) { return V{x, x}; }
-    __device__ static V rows_init(int v_lo, int v_hi) { return V{-v_lo, -v_hi}; }
-    __device__ static V rows_next(V r) { return V{r.l + 1, r.h + 1}; }
-    __device__ static int32_t row_lo(V r) { return r.l; }
-    __device__ static int32_t row_hi(V r) { return r.h; }
+    __device__ static V sel_step(V m, int s, V b) { return V{m.l ? s : b.l, m.h ? s : b.h}; }
+    __device__ static int32_t step_lo(V r) { return r.l; }
+    __device__ static int32_t step_hi(V r) { return r.h; }
     __device__ static V dpp(V x) { return V{(int32_t)row_ror1((uint32_t)x.l), (int32_t)row_ror1((uint32_t)x.h)}; }
-    __device__ static V fix_h(V x, B bnd, uint32_t m)
+    __device__ static V fix_h(V x, B bnd, const LaneFix& f)
     {
-        return V{(int32_t)bfi(m, bnd.x, (uint32_t)x.l), (int32_t)bfi(m, (uint32_t)x.l, (uint32_t)x.h)};
+        return V{(int32_t)bfi(f.mask, bnd.x, (uint32_t)x.l), (int32_t)bfi(f.mask, (uint32_t)x.l, (uint32_t)x.h)};
     }
-    __device__ static V fix_e(V x, B bnd, uint32_t m)
+    __device__ static V fix_e(V x, B bnd, const LaneFix& f)
     {
-        return V{(int32_t)bfi(m, bnd.y, (uint32_t)x.l), (int32_t)bfi(m, (uint32_t)x.l, (uint32_t)x.h)};
+        return V{(int32_t)bfi(f.mask, bnd.y, (uint32_t)x.l), (int32_t)bfi(f.mask, (uint32_t)x.l, (uint32_t)x.h)};
     }
     __device__ static B bnd_out(V h, V e) { return make_uint2((uint32_t)h.h, (uint32_t)e.h); }
     template <int K>
@@ -271,7 +287,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     const int g = lane >> 4, l = lane & 15;
     const int alpha = a.alpha;
     const int rowstride = WA * 2;
-    const uint32_t lane0_mask = l == 0 ? 0xFFFFFFFFu : 0u;
+    const LaneFix lfix = lane_fix(l);
     uint8_t* gb = sm.group(g);
     int16_t* prof = (int16_t*)gb;
     uint16_t* ring_code = (uint16_t*)(gb + sm.L.ring_code_off);
@@ -292,6 +308,9 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         strips_max = max(strips_max, info[q * 8 + 0]);
         steps_max = max(steps_max, info[q * 8 + 1]);
     }
+    // the same in every lane: keep the loop bounds scalar
+    strips_max = __builtin_amdgcn_readfirstlane(strips_max);
+    steps_max = __builtin_amdgcn_readfirstlane(steps_max);
 
     const bool multi_strip = strips_max > 1;
     const V OPEN2 = A::splat(-a.gap_open);
@@ -339,8 +358,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
 #pragma unroll
         for (int k = 0; k < (REV ? 1 : K); ++k) snap[k] = A::zero();
         V Hout = A::zero(), Eout = A::zero(), Hdiag0 = A::zero();
-        V best = A::zero(), brow = A::zero();
-        V rowpk = A::rows_init(v_lo, v_hi); // row of each half at step 0
+        V best = A::zero(), bstep = A::zero(); // bstep: step of the first row reaching best
         int32_t rt_lo = 0x7FFFFFFF, rr_lo = 0, rt_hi = 0x7FFFFFFF, rr_hi = 0; // reverse hits
 
         for (int s0 = 0; s0 < steps_max; s0 += kChunk) {
@@ -360,8 +378,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
                 V xH = A::dpp(Hout), xE = A::dpp(Eout);
                 const Bd bnd = ring_bnd[s & (kRing - 1)]; // zero unless a strip precedes
-                xH = A::fix_h(xH, bnd, lane0_mask);
-                xE = A::fix_e(xE, bnd, lane0_mask);
+                xH = A::fix_h(xH, bnd, lfix);
+                xE = A::fix_e(xE, bnd, lfix);
                 uint32_t plo[KS / 2], phi[KS / 2];
                 prof_load<K>((const uint8_t*)prof + off_lo, v_lo, plo);
                 prof_load<K>((const uint8_t*)prof + off_hi, v_hi, phi);
@@ -393,7 +411,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 if constexpr (!REV) {
                     const V m = A::ltmask(best, rm);
                     best = A::vmax(best, rm);
-                    brow = A::sel(m, rowpk, brow);
+                    bstep = A::sel_step(m, s, bstep);
 #pragma unroll
                     for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
                 } else {
@@ -416,7 +434,6 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                         }
                     }
                 }
-                rowpk = A::rows_next(rowpk);
             };
             // this chunk's row codes of each virtual lane are contiguous in the ring
             const uint16_t* rc_lo = ring_code + ((s0 - v_lo) & (kRing - 1));
@@ -435,7 +452,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             for (int h = 0; h < 2; ++h) {
                 const int32_t b = h ? A::hi(best) : A::lo(best);
                 if (live && b > 0) {
-                    const int32_t row = h ? A::row_hi(brow) : A::row_lo(brow);
+                    const int32_t row = h ? A::step_hi(bstep) - v_hi : A::step_lo(bstep) - v_lo;
                     const int base = strip_base + (h ? v_hi : v_lo) * K;
                     uint32_t pbest = 0xFFFFFFFFu;
 #pragma unroll
