@@ -1,6 +1,6 @@
 // gbsw_binning.hip -- device-side length binning of a batch (batched form).
 //
-// Every pair gets a 20-bit key (bucket of its shorter length) << 16 | (65535 -
+// Every pair gets a 21-bit key (bucket id of its shorter length) << 16 | (65535 -
 // its longer length); a stable radix sort of (key, pair index) yields the work
 // order: buckets in ascending order, and inside a bucket the longest pairs first
 // (longest-processing-time order for the persistent work queue).  The reference

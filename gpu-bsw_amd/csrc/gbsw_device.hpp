@@ -49,6 +49,42 @@ __device__ __forceinline__ uint32_t bfi(uint32_t m, uint32_t a, uint32_t b)
     asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
     return r;
 }
+// ---- integer-valued packed f16 (exact for |x| <= 2048)
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ f16x2 as_h(uint32_t v) { return __builtin_bit_cast(f16x2, v); }
+__device__ __forceinline__ uint32_t raw(f16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+// v_pk_add_f16
+__device__ __forceinline__ uint32_t pk_add_f16(uint32_t a, uint32_t b) { return raw(as_h(a) + as_h(b)); }
+// v_pk_max_f16.  Written as asm: the operands are integers (never NaN), and the
+// builtin would first canonicalise each operand with an extra max.
+__device__ __forceinline__ uint32_t pk_max_f16(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_max_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// v_pk_maximum3_f16 (gfx950): max of three per half.
+__device__ __forceinline__ uint32_t pk_max3_f16(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_pk_maximum3_f16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+// 0xFFFF in each half where a < b, else 0: the sign of a - b (exact, and +0 when
+// equal), spread by v_pk_ashrrev_i16.
+__device__ __forceinline__ uint32_t pk_lt_mask_f16(uint32_t a, uint32_t b)
+{
+    uint32_t d, m;
+    asm("v_pk_add_f16 %0, %1, %2 neg_lo:[0,1] neg_hi:[0,1]" : "=v"(d) : "v"(a), "v"(b));
+    asm("v_pk_ashrrev_i16 %0, 15, %1 op_sel_hi:[0,1]" : "=v"(m) : "v"(d));
+    return m;
+}
+__host__ __device__ inline uint16_t f16_bits(int32_t x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+__device__ __forceinline__ int32_t f16_int(uint32_t bits16)
+{
+    return (int32_t)__builtin_bit_cast(_Float16, (uint16_t)bits16);
+}
+
 // v_bfi_b32 with a wave-uniform (SGPR) second operand.
 __device__ __forceinline__ uint32_t bfi_s(uint32_t m, uint32_t a, uint32_t b)
 {

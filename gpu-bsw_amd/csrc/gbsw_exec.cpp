@@ -63,6 +63,16 @@ hipError_t PinnedBuffer::reserve(size_t b)
 
 namespace {
 
+// GBSW_F16=0 keeps every bucket on the int16 kernels (comparison runs).
+bool f16_disabled()
+{
+    static const bool off = [] {
+        const char* e = std::getenv("GBSW_F16");
+        return e && std::atoi(e) == 0;
+    }();
+    return off;
+}
+
 // Streams used for the length buckets: 1 + kAuxStreams by default;
 // GBSW_STREAMS=1 serialises the buckets (per-kernel timing and counters).
 int bucket_streams()
@@ -205,6 +215,13 @@ int k_for_cols(int cols, int kmax)
     return best;
 }
 
+int bucket_ids()
+{
+    int nk = 0;
+    kernel_k_list(&nk);
+    return 2 * nk;
+}
+
 const std::vector<uint8_t>& bucket_table(int kmax)
 {
     static std::mutex mu;
@@ -219,7 +236,10 @@ const std::vector<uint8_t>& bucket_table(int kmax)
         const int K = k_for_cols(m, kmax);
         int idx = 0;
         while (idx < nk && ks[idx] != K) ++idx;
-        t[(size_t)m] = (uint8_t)idx;
+        // pairs spanning several strips get buckets of their own: a single-strip
+        // bucket then bounds its scores by 32 x K columns
+        const bool multi = m > kVirtPerPair * K;
+        t[(size_t)m] = (uint8_t)(idx + (multi ? nk : 0));
     }
     return cache.emplace(kmax, std::move(t)).first->second;
 }
@@ -230,22 +250,27 @@ void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry)
     const std::vector<uint8_t>& tab = bucket_table(kmax);
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
-    std::vector<int64_t> count((size_t)nk, 0), cells((size_t)nk, 0);
+    const int nb = bucket_ids();
+    std::vector<int64_t> count((size_t)nb, 0), cells((size_t)nb, 0);
+    std::vector<int32_t> mcols((size_t)nb, 0);
     for (int64_t p = 0; p < job.n; ++p) {
         const int32_t la = len_ref[p], lb = len_qry[p];
-        const int bi = tab[(size_t)std::min<int32_t>(std::min(la, lb), 65535)];
+        const int32_t m = std::min(la, lb);
+        const int bi = tab[(size_t)std::min<int32_t>(m, 65535)];
         count[(size_t)bi] += 1;
         cells[(size_t)bi] += (int64_t)la * lb;
+        mcols[(size_t)bi] = std::max(mcols[(size_t)bi], m);
     }
     job.buckets.clear();
     int64_t at = 0;
-    for (int i = 0; i < nk; ++i) {
+    for (int i = 0; i < nb; ++i) {
         if (count[(size_t)i] == 0) continue;
         Bucket bk;
-        bk.K = ks[i];
+        bk.K = ks[i % nk];
         bk.begin = at;
         bk.count = count[(size_t)i];
         bk.cells = cells[(size_t)i];
+        bk.max_cols = mcols[(size_t)i];
         at += bk.count;
         job.buckets.push_back(bk);
     }
@@ -253,14 +278,28 @@ void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry)
 
 int batched_grid_cap(const Exec& ex, int K, int alpha)
 {
-    const int occ = kernel_occupancy(kBatched, K, lds_for(K, alpha));
+    // the f16 and int16 kernels of a geometry share its LDS layout; take the
+    // smaller occupancy so the bound holds for either
+    const int lds = lds_for(K, alpha);
+    int occ = kernel_occupancy(kBatched, K, lds);
+    if (kernel_has(kBatchedF16, K)) occ = std::min(occ, kernel_occupancy(kBatchedF16, K, lds));
     return std::max(1, occ) * ex.num_cu;
+}
+
+int batched_kind(const Bucket& bk, int32_t max_sub)
+{
+    // every cell value of a pair is at most its score plus one substitution, and
+    // the score at most (shorter length) x (best substitution)
+    const int64_t bound = (int64_t)bk.max_cols * max_sub + max_sub;
+    if (kernel_has(kBatchedF16, bk.K) && bound < kF16Exact && !f16_disabled()) return kBatchedF16;
+    return kBatched;
 }
 
 int sort_scratch_bytes(int64_t n, size_t* bytes)
 {
     size_t tb = 0;
-    GBSW_HIP_TRY(sort_pairs(nullptr, tb, nullptr, nullptr, nullptr, nullptr, (int)std::max<int64_t>(1, n), 20, nullptr));
+    GBSW_HIP_TRY(sort_pairs(nullptr, tb, nullptr, nullptr, nullptr, nullptr, (int)std::max<int64_t>(1, n), kSortBits,
+                            nullptr));
     *bytes = std::max<size_t>(tb, 16);
     return GBSW_OK;
 }
@@ -343,7 +382,8 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         // device binning: keys, then a stable radix sort into the work order
         GBSW_HIP_TRY(bin_keys(job.ref_off, job.qry_off, job.n, job.bucket_of_len, job.keys, job.vals, st0));
         size_t tb = job.sort_temp_bytes;
-        GBSW_HIP_TRY(sort_pairs(job.sort_temp, tb, job.keys, job.keys_sorted, job.vals, job.order, (int)job.n, 20, st0));
+        GBSW_HIP_TRY(sort_pairs(job.sort_temp, tb, job.keys, job.keys_sorted, job.vals, job.order, (int)job.n,
+                                kSortBits, st0));
         const size_t nb = job.buckets.size();
         if (ex.queue.bytes < sizeof(uint32_t) * nb) GBSW_HIP_TRY(ex.queue.reserve(sizeof(uint32_t) * nb));
         GBSW_HIP_TRY(zero_u32(ex.queue.as<uint32_t>(), (int)nb, st0));
@@ -351,13 +391,14 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         // work item takes longest (largest K) first, so a bucket with few long
         // pairs overlaps the bulk instead of running alone at the end.
         std::vector<size_t> order(nb);
-        std::vector<int> grids(nb), ldss(nb);
+        std::vector<int> grids(nb), ldss(nb), kinds(nb);
         size_t bnd_words = 0;
         for (size_t i = 0; i < nb; ++i) {
             order[i] = i;
             const Bucket& bk = job.buckets[i];
+            kinds[i] = batched_kind(bk, max_sub);
             ldss[i] = lds_for(bk.K, job.alpha);
-            grids[i] = grid_for(ex, kBatched, bk.K, ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
+            grids[i] = grid_for(ex, kinds[i], bk.K, ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
             bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
         }
         std::sort(order.begin(), order.end(),
@@ -373,8 +414,9 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
             a.order = job.order + bk.begin;
             a.order_n = bk.count;
             a.queue = ex.queue.as<uint32_t>() + i;
+            a.sat_limit = kinds[i] == kBatchedF16 ? kF16Exact - max_sub : 32767 - max_sub;
             hipStream_t st = (j % ns) == 0 ? st0 : ex.aux[j % ns - 1];
-            rc = launch_timed(ex, st, kBatched, bk.K, grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
+            rc = launch_timed(ex, st, kinds[i], bk.K, grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
             if (rc) return rc;
             ex.launches.back().pairs = bk.count;
             ex.launches.back().cells = bk.cells;

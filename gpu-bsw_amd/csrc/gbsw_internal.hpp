@@ -52,6 +52,7 @@ struct Bucket {
     int K = 0;
     int64_t begin = 0, count = 0;
     int64_t cells = 0;
+    int32_t max_cols = 0; // longest shorter sequence: bounds the bucket's scores
 };
 
 struct LaunchRec {
@@ -107,14 +108,21 @@ struct Job {
 int max_k_for(int alpha);
 // K for a shorter sequence of `cols` residues (strips x (K + 2) minimised).
 int k_for_cols(int cols, int kmax);
-// Host table shorter-length -> bucket index (position of its K in kernel_k_list),
-// 65536 entries, cached per kmax.
+// Bucket ids: position of K in kernel_k_list, plus the list's length for pairs
+// that span several strips (bucket_ids() in all).
+int bucket_ids();
+// Host table shorter-length -> bucket id, 65536 entries, cached per kmax.
 const std::vector<uint8_t>& bucket_table(int kmax);
+// Key bits of the device radix sort: bucket id << 16 | (65535 - longer length).
+constexpr int kSortBits = 21;
 // Fill job.buckets from per-pair lengths (host), ordered by ascending K.
 void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry);
 // Workgroups of a full-occupancy batched launch for K columns per lane (the grid
 // of a bucket is min(this, ceil(pairs / kPairsPerWave))).
 int batched_grid_cap(const Exec& ex, int K, int alpha);
+// Kernel of a batched bucket: the f16 kernel when its score bound allows, else
+// the int16 one.
+int batched_kind(const Bucket& bk, int32_t max_sub);
 // Scratch bytes of the radix sort for n pairs.
 int sort_scratch_bytes(int64_t n, size_t* bytes);
 

@@ -106,13 +106,50 @@ struct Packed16 {
     }
     __device__ static V add(V a, V b) { return pk_add_sat(a, b); }
     __device__ static V vmax(V a, V b) { return pk_max(a, b); }
+    __device__ static V vmax3(V a, V b, V c) { return pk_max(pk_max(a, b), c); }
     __device__ static V subf0(V a, V b) { return pk_sub_floor0(a, b); }
+    // gap step: operand g(score) and the three uses of the recurrence
+    __device__ static V gap(int32_t score) { return splat(-score); }
+    __device__ static V cell(V D, V E, V F) { return pk_max(pk_max(D, E), F); }   // H
+    __device__ static V open_in(V H, V G) { return pk_sub_floor0(H, G); }          // max(H + open, 0)
+    __device__ static V decay(V X, V G, V Ho) { return pk_max(pk_sub_floor0(X, G), Ho); }
     __device__ static V ltmask(V a, V b) { return pk_lt_mask(a, b); }
     __device__ static V sel(V m, V a, V b) { return bfi(m, a, b); }
     __device__ static int32_t lo(V x) { return lo16(x); }
     __device__ static int32_t hi(V x) { return hi16(x); }
+    // equality with a splat value, on the raw halves
+    __device__ static bool eq_lo(V x, V t) { return (uint16_t)x == (uint16_t)t; }
+    __device__ static bool eq_hi(V x, V t) { return (uint16_t)(x >> 16) == (uint16_t)(t >> 16); }
+    __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
     __device__ static B load_bnd(const B* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
     __device__ static B zero_bnd() { return 0u; }
+};
+
+// PackedF16: the Packed16 layout with each half an integer-valued f16, exact
+// while every value stays within +-2048.  The host uses it only for length
+// buckets whose largest possible score (shorter length x best substitution)
+// plus one substitution is below 2048; there H = max(D, E, F) and the max over a
+// lane's columns take one v_pk_maximum3_f16 per three operands.  E and F keep
+// their floor at 0 through the third operand, so every cell value equals the
+// Packed16 one.  Very negative D values (padding, -16384 + H) are rounded but
+// never selected, since E, F >= 0.
+struct PackedF16 : Packed16 {
+    __device__ static V splat(int32_t x)
+    {
+        const uint32_t b = f16_bits(x);
+        return b | (b << 16);
+    }
+    __device__ static V add(V a, V b) { return pk_add_f16(a, b); }
+    __device__ static V vmax(V a, V b) { return pk_max_f16(a, b); }
+    __device__ static V vmax3(V a, V b, V c) { return pk_max3_f16(a, b, c); }
+    __device__ static V gap(int32_t score) { return splat(score); }
+    __device__ static V cell(V D, V E, V F) { return pk_max3_f16(D, E, F); }
+    __device__ static V open_in(V H, V G) { return pk_add_f16(H, G); } // may be < 0: decay floors it
+    __device__ static V decay(V X, V G, V Ho) { return pk_max3_f16(pk_add_f16(X, G), Ho, 0u); }
+    __device__ static V ltmask(V a, V b) { return pk_lt_mask_f16(a, b); }
+    __device__ static int32_t lo(V x) { return f16_int(x & 0xFFFFu); }
+    __device__ static int32_t hi(V x) { return f16_int(x >> 16); }
+    __device__ static int16_t prof_val(int32_t s) { return (int16_t)f16_bits(s); }
 };
 
 // Wide32: the same two virtual lanes as two int32 VGPRs, for pairs whose score
@@ -146,11 +183,19 @@ struct Wide32 {
     }
     __device__ static V add(V a, V b) { return V{a.l + b.l, a.h + b.h}; }
     __device__ static V vmax(V a, V b) { return V{max(a.l, b.l), max(a.h, b.h)}; }
+    __device__ static V vmax3(V a, V b, V c) { return vmax(vmax(a, b), c); }
     __device__ static V subf0(V a, V b)
     {
         return V{(int32_t)__builtin_elementwise_sub_sat((uint32_t)a.l, (uint32_t)b.l),
                  (int32_t)__builtin_elementwise_sub_sat((uint32_t)a.h, (uint32_t)b.h)};
     }
+    __device__ static V gap(int32_t score) { return splat(-score); }
+    __device__ static V cell(V D, V E, V F) { return vmax(vmax(D, E), F); }
+    __device__ static V open_in(V H, V G) { return subf0(H, G); }
+    __device__ static V decay(V X, V G, V Ho) { return vmax(subf0(X, G), Ho); }
+    __device__ static bool eq_lo(V x, V t) { return x.l == t.l; }
+    __device__ static bool eq_hi(V x, V t) { return x.h == t.h; }
+    __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
     __device__ static V ltmask(V a, V b) { return V{a.l < b.l ? -1 : 0, a.h < b.h ? -1 : 0}; }
     __device__ static V sel(V m, V a, V b)
     {
@@ -254,13 +299,15 @@ __device__ __forceinline__ void prof_slot(int j, int& v, int& k)
 template <class A, int K>
 __device__ __forceinline__ typename A::V tree_max(const typename A::V (&h)[K])
 {
+    // ternary tree: ceil((K - 1) / 2) three-input maxima
     typename A::V t[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) t[k] = h[k];
 #pragma unroll
-    for (int w = 1; w < K; w *= 2)
+    for (int w = 1; w < K; w *= 3)
 #pragma unroll
-        for (int k = 0; k + w < K; k += 2 * w) t[k] = A::vmax(t[k], t[k + w]);
+        for (int k = 0; k + w < K; k += 3 * w)
+            t[k] = (k + 2 * w < K) ? A::vmax3(t[k], t[k + w], t[k + 2 * w]) : A::vmax(t[k], t[k + w]);
     return t[0];
 }
 
@@ -313,8 +360,9 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     steps_max = __builtin_amdgcn_readfirstlane(steps_max);
 
     const bool multi_strip = strips_max > 1;
-    const V OPEN2 = A::splat(-a.gap_open);
-    const V EXT2 = A::splat(-a.gap_extend);
+    const V OPEN2 = A::gap(a.gap_open);
+    const V EXT2 = A::gap(a.gap_extend);
+    const V TGT = A::splat(target); // reverse pass: the forward score
     const int v_lo = l, v_hi = l + 16;
     uint64_t cand_lo = 0, cand_hi = 0;
 
@@ -336,7 +384,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 if (code >= alpha) code = -1;
             }
             for (int r = 0; r <= alpha; ++r)
-                prof[r * WA + j] = (code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg;
+                prof[r * WA + j] = A::prof_val((code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg);
         }
         // ---- row ring: rows [-32, 0) padding, [0, 32) now, [32, 64) in flight
         {
@@ -390,10 +438,10 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                     const V S = A::template prof<K>(plo, phi, k);
                     const V D = A::add(Hd, S);
                     Hd = Hcol[k];
-                    const V H = A::vmax(A::vmax(D, E), Fcol[k]);
-                    const V Ho = A::subf0(H, OPEN2);
-                    E = A::vmax(A::subf0(E, EXT2), Ho);
-                    Fcol[k] = A::vmax(A::subf0(Fcol[k], EXT2), Ho);
+                    const V H = A::cell(D, E, Fcol[k]);
+                    const V Ho = A::open_in(H, OPEN2);
+                    E = A::decay(E, EXT2, Ho);
+                    Fcol[k] = A::decay(Fcol[k], EXT2, Ho);
                     Hcol[k] = H;
                 }
                 Hout = Hcol[K - 1];
@@ -415,14 +463,14 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
 #pragma unroll
                     for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
                 } else {
-                    if (A::lo(rm) == target || A::hi(rm) == target) {
+                    if (A::eq_lo(rm, TGT) || A::eq_hi(rm, TGT)) {
                         const int row_lo = s - v_lo, row_hi = s - v_hi;
                         // first (smallest) column of this row that reaches the target
                         int k_lo = K, k_hi = K;
 #pragma unroll
                         for (int k = K - 1; k >= 0; --k) {
-                            if (A::lo(Hcol[k]) == target && strip_base + v_lo * K + k < t.nc) k_lo = k;
-                            if (A::hi(Hcol[k]) == target && strip_base + v_hi * K + k < t.nc) k_hi = k;
+                            if (A::eq_lo(Hcol[k], TGT) && strip_base + v_lo * K + k < t.nc) k_lo = k;
+                            if (A::eq_hi(Hcol[k], TGT) && strip_base + v_hi * K + k < t.nc) k_hi = k;
                         }
                         if (k_lo < K && row_lo >= 0 && row_lo < t.nr && strip_base + v_lo * K + k_lo < rt_lo) {
                             rt_lo = strip_base + v_lo * K + k_lo;
@@ -779,9 +827,19 @@ static const void* wide_ptr()
     return (const void*)&k_batched<Wide32, K>;
 }
 
+template <int K>
+static const void* f16_ptr()
+{
+    return (const void*)&k_batched<PackedF16, K>;
+}
+
 #define GBSW_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
 
 #define GBSW_WIDE_K_LIST(X) X(4) X(8) X(16)
+
+// f16 kernels: the geometries whose single strip (32 x K columns) keeps DNA-like
+// scores below 2048
+#define GBSW_F16_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20)
 
 static const void* kernel_for(int kind, int K)
 {
@@ -789,6 +847,14 @@ static const void* kernel_for(int kind, int K)
         switch (K) {
 #define GBSW_CASE(k) case k: return wide_ptr<k>();
             GBSW_WIDE_K_LIST(GBSW_CASE)
+#undef GBSW_CASE
+        default: return nullptr;
+        }
+    }
+    if (kind == kBatchedF16) {
+        switch (K) {
+#define GBSW_CASE(k) case k: return f16_ptr<k>();
+            GBSW_F16_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
         default: return nullptr;
         }
@@ -802,6 +868,7 @@ static const void* kernel_for(int kind, int K)
 }
 
 bool kernel_supported(int K) { return kernel_for(kPlainForward, K) != nullptr; }
+bool kernel_has(int kind, int K) { return kernel_for(kind, K) != nullptr; }
 
 const int* wide_k_list(int* count)
 {

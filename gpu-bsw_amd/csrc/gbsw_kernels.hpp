@@ -6,9 +6,13 @@
 
 namespace gbsw {
 
-enum KernelKind { kPlainForward = 0, kPlainReverse = 1, kBatched = 2, kBatchedWide = 3 };
+enum KernelKind { kPlainForward = 0, kPlainReverse = 1, kBatched = 2, kBatchedWide = 3, kBatchedF16 = 4 };
+
+// largest score + one substitution an f16 kernel may meet (exact integers in f16)
+constexpr int kF16Exact = 2048;
 
 bool kernel_supported(int K);
+bool kernel_has(int kind, int K);
 const int* kernel_k_list(int* count); // K values with compiled kernels, ascending
 const int* wide_k_list(int* count);   // K values of the int32 (kBatchedWide) kernel
 int kernel_occupancy(int kind, int K, int lds_bytes);

@@ -166,6 +166,7 @@ struct Request {
     int32_t max_rows = 0, max_cols = 0;
     int64_t bucket_count[kMaxBuckets] = {0};
     int64_t bucket_cells[kMaxBuckets] = {0};
+    int32_t bucket_maxcols[kMaxBuckets] = {0};
     // progress (guarded by the server mutex)
     int64_t next = 0;
     int parts = 0, parts_pending = 0;
@@ -281,10 +282,14 @@ int gbsw_server::init_tables(int which)
     t.bucket_of_len = bucket_table(t.kmax);
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
-    if (nk > kMaxBuckets) return fail(GBSW_E_INTERNAL, "too many kernel geometries");
-    t.bucket_k.assign(ks, ks + nk);
-    t.bucket_full.resize((size_t)nk);
-    for (int i = 0; i < nk; ++i) t.bucket_full[(size_t)i] = batched_grid_cap(slots[0]->ex, ks[i], t.alpha);
+    const int nb = bucket_ids();
+    if (nb > kMaxBuckets) return fail(GBSW_E_INTERNAL, "too many kernel geometries");
+    t.bucket_k.resize((size_t)nb);
+    t.bucket_full.resize((size_t)nb);
+    for (int i = 0; i < nb; ++i) {
+        t.bucket_k[(size_t)i] = ks[i % nk];
+        t.bucket_full[(size_t)i] = batched_grid_cap(slots[0]->ex, ks[i % nk], t.alpha);
+    }
     int rc = alloc_into(t.d_bucket_of_len, 65536, device_bytes);
     if (!rc) rc = alloc_into(t.d_accept, 256, device_bytes);
     if (rc) return rc;
@@ -343,6 +348,7 @@ void gbsw_server::admit(Slot& s)
     const int64_t cap_pairs = cfg.pairs_per_slot, cap_res = cfg.residues_per_slot;
     const int64_t cap_bnd = cfg.boundary_bytes_per_slot;
     int64_t cnt[kMaxBuckets] = {0}, cel[kMaxBuckets] = {0};
+    int32_t mcol[kMaxBuckets] = {0};
     int64_t pairs = 0, res = 0, quads = 0, cells = 0;
     int32_t max_rows = 0, max_cols = 0;
     auto quads_of = [&](int b, int64_t count) {
@@ -365,6 +371,7 @@ void gbsw_server::admit(Slot& s)
                 for (int b = 0; b < nb; ++b) {
                     cnt[b] += r->bucket_count[b];
                     cel[b] += r->bucket_cells[b];
+                    mcol[b] = std::max(mcol[b], r->bucket_maxcols[b]);
                 }
                 quads = q2;
                 max_rows = rows2;
@@ -388,6 +395,7 @@ void gbsw_server::admit(Slot& s)
                 }
                 cnt[b] += 1;
                 cel[b] += (int64_t)la * lb;
+                mcol[b] = std::max(mcol[b], std::min(la, lb));
                 quads = q2;
                 max_rows = rows2;
                 max_cols = std::max(max_cols, std::min(la, lb));
@@ -437,6 +445,7 @@ void gbsw_server::admit(Slot& s)
         bk.begin = at;
         bk.count = cnt[b];
         bk.cells = cel[b];
+        bk.max_cols = mcol[b];
         at += cnt[b];
         j.buckets.push_back(bk);
     }
@@ -794,6 +803,7 @@ int gbsw_server_submit(gbsw_server* srv, const gbsw_scoring* sc, int64_t n, cons
         const int b = T.bucket_of_len[(size_t)std::min(la, lb)];
         r->bucket_count[b] += 1;
         r->bucket_cells[b] += (int64_t)(la * lb);
+        r->bucket_maxcols[b] = std::max<int32_t>(r->bucket_maxcols[b], (int32_t)std::min(la, lb));
         r->cells += (int64_t)(la * lb);
         r->max_rows = std::max<int32_t>(r->max_rows, (int32_t)std::max(la, lb));
         r->max_cols = std::max<int32_t>(r->max_cols, (int32_t)std::min(la, lb));
