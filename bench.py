@@ -41,8 +41,25 @@ MI355X_CU, MI355X_GHZ, OPS_PER_CELL = 256, 2.4, 12
 PEAK_INT_TOPS_PACKED = MI355X_CU * 64 * MI355X_GHZ * 2 / 1000.0  # 78.6 TOP/s (int16x2)
 
 
+KIND_NAMES = {2: "Packed16", 3: "Wide32", 4: "PackedF16"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def profiled_traffic(kname):
+    """HBM bytes per launch of `kname` (FETCH_SIZE + WRITE_SIZE, calibrated) from
+    the committed rocprofv3 summary, or None when it does not cover this kernel."""
+    import re
+    path = os.path.join(REPO, "profiles", "k_batched_dna_mix_1m.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes_per_launch\s+(.+?)\s*=\s*([0-9.eE+]+)", line)
+        if m and m.group(1).strip() == kname:
+            return float(m.group(2))
+    return None
 
 
 def cpu_baseline(w, sample_pairs, threads):
@@ -183,16 +200,25 @@ def main():
             log("ERROR: plain and batched results differ")
             sys.exit(1)
 
-    # dominant kernel: the batched launch (one length bucket, persistent, forward +
-    # reverse fused) with the longest average duration; algorithmic work per
-    # launch = 12 integer ops x that bucket's forward cells (|ref|*|query|)
-    dom = max(batched["launches"], key=lambda x: x["ms"])
+    # Dominant kernel.  The timed steps run the length buckets on four streams, so
+    # their launches share the GPU and a launch's duration is not its own.  One
+    # more step runs the buckets one after another: the launch with the longest
+    # duration there is the dominant kernel (one length bucket, persistent, forward
+    # and reverse fused); its algorithmic work = 12 integer ops x the bucket's
+    # forward cells (|ref| x |query|).
+    ctx.set_option(gbsw.OPT_STREAMS, 1)
+    serial = time_mode(batch, scoring, "batched", 1, 1, cells)
+    ctx.set_option(gbsw.OPT_STREAMS, 4)
+    dom = max(serial["launches"], key=lambda x: x["ms"])
+    kname = "k_batched<%s, %d>" % (KIND_NAMES.get(dom["kind"], "?"), dom["columns_per_lane"])
     achieved = OPS_PER_CELL * dom["cells"] / (dom["ms"] / 1000.0) / 1e12
     agg = OPS_PER_CELL * cells / (batched["device_ms"] / 1000.0) / 1e12
+    log("dominant kernel (buckets serialised, %.2f ms/step): %s, %d pairs, %.4g cells, %.3f ms, %.1f GCUPS" %
+        (serial["device_ms"], kname, dom["pairs"], dom["cells"], dom["ms"], dom["cells"] / dom["ms"] / 1e6))
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_INT_TOPS_PACKED, 1),
-                "unit": "TOP/s", "frac": round(achieved / PEAK_INT_TOPS_PACKED, 4), "traffic": None,
-                "kernel": "k_batched<%d> (bucket of %d pairs, %.4g cells)" % (dom["columns_per_lane"], dom["pairs"],
-                                                                              dom["cells"]),
+                "unit": "TOP/s", "frac": round(achieved / PEAK_INT_TOPS_PACKED, 4),
+                "traffic": profiled_traffic(kname),
+                "kernel": "%s (bucket of %d pairs, %.4g cells)" % (kname, dom["pairs"], dom["cells"]),
                 "kernel_ms": round(dom["ms"], 3), "ops_per_cell": OPS_PER_CELL,
                 "aggregate_achieved": round(agg, 3), "aggregate_frac": round(agg / PEAK_INT_TOPS_PACKED, 4)}
 

@@ -115,6 +115,23 @@ int gbsw_context_create(int32_t device, gbsw_context** out)
     return GBSW_OK;
 }
 
+int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value)
+{
+    if (!ctx) return fail(GBSW_E_ARGUMENT, "context is null");
+    switch (option) {
+    case GBSW_OPT_STREAMS:
+        if (value < 1 || value > kAuxStreams + 1) return fail(GBSW_E_ARGUMENT, "streams must be 1..4");
+        ctx->ex.streams = (int)value;
+        return GBSW_OK;
+    case GBSW_OPT_F16:
+        if (value != 0 && value != 1) return fail(GBSW_E_ARGUMENT, "f16 option is 0 or 1");
+        ctx->ex.f16 = value != 0;
+        return GBSW_OK;
+    default:
+        return fail(GBSW_E_ARGUMENT, "unknown option");
+    }
+}
+
 int gbsw_context_destroy(gbsw_context* ctx)
 {
     if (!ctx) return GBSW_OK;

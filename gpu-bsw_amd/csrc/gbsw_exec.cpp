@@ -63,26 +63,20 @@ hipError_t PinnedBuffer::reserve(size_t b)
 
 namespace {
 
-// GBSW_F16=0 keeps every bucket on the int16 kernels (comparison runs).
-bool f16_disabled()
+// Defaults of the per-context options.  GBSW_F16=0 keeps every bucket on the
+// int16 kernels; GBSW_STREAMS=1 serialises the buckets (per-kernel timing and
+// counters).
+bool default_f16()
 {
-    static const bool off = [] {
-        const char* e = std::getenv("GBSW_F16");
-        return e && std::atoi(e) == 0;
-    }();
-    return off;
+    const char* e = std::getenv("GBSW_F16");
+    return !(e && std::atoi(e) == 0);
 }
 
-// Streams used for the length buckets: 1 + kAuxStreams by default;
-// GBSW_STREAMS=1 serialises the buckets (per-kernel timing and counters).
-int bucket_streams()
+int default_streams()
 {
-    static const int n = [] {
-        const char* e = std::getenv("GBSW_STREAMS");
-        const int v = e ? std::atoi(e) : kAuxStreams + 1;
-        return std::max(1, std::min(kAuxStreams + 1, v));
-    }();
-    return n;
+    const char* e = std::getenv("GBSW_STREAMS");
+    const int v = e ? std::atoi(e) : kAuxStreams + 1;
+    return std::max(1, std::min(kAuxStreams + 1, v));
 }
 
 // LDS bytes of a workgroup; bnd_bytes = 4 (packed int16) or 8 (int32 kernel).
@@ -146,6 +140,8 @@ int Exec::init(int dev, int ncu)
 {
     device = dev;
     num_cu = ncu;
+    streams = default_streams();
+    f16 = default_f16();
     hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
     if (e == hipSuccess) e = hipEventCreate(&ev_stop);
@@ -286,12 +282,12 @@ int batched_grid_cap(const Exec& ex, int K, int alpha)
     return std::max(1, occ) * ex.num_cu;
 }
 
-int batched_kind(const Bucket& bk, int32_t max_sub)
+int batched_kind(const Bucket& bk, int32_t max_sub, bool allow_f16)
 {
     // every cell value of a pair is at most its score plus one substitution, and
     // the score at most (shorter length) x (best substitution)
     const int64_t bound = (int64_t)bk.max_cols * max_sub + max_sub;
-    if (kernel_has(kBatchedF16, bk.K) && bound < kF16Exact && !f16_disabled()) return kBatchedF16;
+    if (allow_f16 && kernel_has(kBatchedF16, bk.K) && bound < kF16Exact) return kBatchedF16;
     return kBatched;
 }
 
@@ -396,7 +392,7 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         for (size_t i = 0; i < nb; ++i) {
             order[i] = i;
             const Bucket& bk = job.buckets[i];
-            kinds[i] = batched_kind(bk, max_sub);
+            kinds[i] = batched_kind(bk, max_sub, ex.f16);
             ldss[i] = lds_for(bk.K, job.alpha);
             grids[i] = grid_for(ex, kinds[i], bk.K, ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
             bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
@@ -404,7 +400,7 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         std::sort(order.begin(), order.end(),
                   [&](size_t x, size_t y) { return job.buckets[x].K > job.buckets[y].K; });
         GBSW_HIP_TRY(ex.bnd.reserve(std::max<size_t>(bnd_words, 1) * sizeof(uint32_t)));
-        const int ns = bucket_streams();
+        const int ns = std::max(1, std::min(kAuxStreams + 1, ex.streams));
         GBSW_HIP_TRY(hipEventRecord(ex.ev_fork, st0));
         for (int i = 0; i < ns - 1; ++i) GBSW_HIP_TRY(hipStreamWaitEvent(ex.aux[i], ex.ev_fork, 0));
         size_t bnd_at = 0;

@@ -77,6 +77,8 @@ struct Exec {
     PinnedBuffer tables_host;    // staging of sub / lut / colmax for the async copies
     std::vector<LaunchRec> launches;
     int32_t max_sub = 0;
+    int streams = kAuxStreams + 1; // streams for the length buckets (1 = serialised)
+    bool f16 = true;               // exact f16 kernels where the score bound allows
     int init(int device, int num_cu);
     void destroy();
 };
@@ -122,7 +124,7 @@ void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry);
 int batched_grid_cap(const Exec& ex, int K, int alpha);
 // Kernel of a batched bucket: the f16 kernel when its score bound allows, else
 // the int16 one.
-int batched_kind(const Bucket& bk, int32_t max_sub);
+int batched_kind(const Bucket& bk, int32_t max_sub, bool allow_f16);
 // Scratch bytes of the radix sort for n pairs.
 int sort_scratch_bytes(int64_t n, size_t* bytes);
 

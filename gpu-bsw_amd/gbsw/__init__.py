@@ -110,7 +110,9 @@ EXPORTED = (
     "gbsw_batch_stats", "gbsw_batch_launch_stats", "gbsw_align", "gbsw_device_memory", "gbsw_version",
     "gbsw_server_config_default", "gbsw_server_create", "gbsw_server_destroy", "gbsw_server_submit",
     "gbsw_server_wait", "gbsw_server_poll", "gbsw_server_stats", "gbsw_host_alloc", "gbsw_host_free",
+    "gbsw_context_set_option",
 )
+OPT_STREAMS, OPT_F16 = 1, 2
 
 _lib = None
 
@@ -131,6 +133,7 @@ def load_library():
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
     lib.gbsw_context_create.argtypes = [i32, ctypes.POINTER(vp)]
     lib.gbsw_context_destroy.argtypes = [vp]
+    lib.gbsw_context_set_option.argtypes = [vp, i32, i64]
     lib.gbsw_last_error.restype = ctypes.c_char_p
     lib.gbsw_version.restype = ctypes.c_char_p
     lib.gbsw_batch_create.argtypes = [vp, i32, i64, vp, vp, vp, vp, ctypes.POINTER(vp)]
@@ -211,6 +214,10 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_option(self, option, value):
+        """OPT_STREAMS (1..4) or OPT_F16 (0/1); see include/gbsw.h."""
+        _check(self._lib.gbsw_context_set_option(self._h, option, int(value)))
 
     def batch(self, alphabet, ref, ref_off, query, query_off):
         return Batch(self, alphabet, ref, ref_off, query, query_off)

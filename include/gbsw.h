@@ -73,6 +73,17 @@ typedef struct {
 int gbsw_context_create(int32_t device, gbsw_context** out);
 int gbsw_context_destroy(gbsw_context* ctx);
 
+/* Context options (gbsw_context_set_option).
+ * STREAMS: streams the length buckets of the batched form run on, 1..4 (default
+ *          4, or the GBSW_STREAMS environment variable); 1 serialises them, so
+ *          every launch is timed alone.
+ * F16:     1 (default; GBSW_F16=0 turns it off): length buckets whose scores
+ *          provably stay below 2048 run an exact integer-valued f16 kernel;
+ *          0: every bucket runs the int16 kernel.  Results are identical. */
+#define GBSW_OPT_STREAMS 1
+#define GBSW_OPT_F16 2
+int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value);
+
 /* Message of the last error on this thread ("" if none). */
 const char* gbsw_last_error(void);
 

@@ -246,3 +246,23 @@ def test_dropin_cpp_caller(tmp_path):
     outa = [l for l in subprocess.run([exe, "aa"], capture_output=True, text=True, check=True).stdout.splitlines()
             if "\t" in l]
     assert outa[1].split("\t")[0] == "33"      # WWW vs AWWWA: 3 x 11
+
+
+def test_batched_options_give_identical_results(ctx):
+    """Serialised or concurrent buckets, f16 kernels on or off: the same results
+    (the f16 kernels run only where their exact range holds)."""
+    for w in (synth.batched_dna(3000, seed=0x5EED0202), synth.plain_protein(600, seed=0x5EED0203)):
+        exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring)
+        sc = gbsw_scoring(w.alphabet, w.scoring)
+        try:
+            for streams, f16 in ((4, 1), (1, 1), (4, 0), (1, 0)):
+                print("case options", w.name, "streams", streams, "f16", f16, flush=True)
+                ctx.set_option(gbsw.OPT_STREAMS, streams)
+                ctx.set_option(gbsw.OPT_F16, f16)
+                got = ctx.align_packed(w.ref, w.ref_off, w.query, w.query_off, sc, "batched")
+                assert_same(got, exp, "%s streams=%d f16=%d" % (w.name, streams, f16))
+        finally:
+            ctx.set_option(gbsw.OPT_STREAMS, 4)
+            ctx.set_option(gbsw.OPT_F16, 1)
+    with pytest.raises(gbsw.GbswError):
+        ctx.set_option(gbsw.OPT_STREAMS, 9)
