@@ -10,10 +10,12 @@ constexpr int kPairsPerWave = 4;
 constexpr int kWave = 64;
 constexpr int kRing = 128;   // row-stream ring entries per pair
 constexpr int kChunk = 32;   // rows refilled at a time
-// The row-code ring keeps a copy of its first kChunk entries after its end, so
-// the 32 consecutive entries a virtual lane reads during one chunk are
-// contiguous whatever the chunk's position in the ring.
-constexpr int kRingCodes = kRing + kChunk;
+// The row-code ring keeps a copy of its first kRingTail entries after its end,
+// so the entries a virtual lane reads during one chunk (32 steps, one step more
+// when the step count is odd, and the look-ahead of one step) are contiguous
+// whatever the chunk's position in the ring.
+constexpr int kRingTail = kChunk + 2;
+constexpr int kRingCodes = kRing + kRingTail;
 constexpr int kMaxAlpha = 31; // codes 0..A-1, row A is the padding row
 constexpr int16_t kNeg = -16384;
 

@@ -120,6 +120,12 @@ struct Packed16 {
     // equality with a splat value, on the raw halves
     __device__ static bool eq_lo(V x, V t) { return (uint16_t)x == (uint16_t)t; }
     __device__ static bool eq_hi(V x, V t) { return (uint16_t)(x >> 16) == (uint16_t)(t >> 16); }
+    // 1 in each half equal to t's (raw bits), else 0
+    __device__ static uint32_t eq_bits(V x, V t)
+    {
+        const u16x2 one = {1, 1};
+        return raw(one - __builtin_elementwise_min(as_u(x ^ t), one));
+    }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
     __device__ static B load_bnd(const B* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
     __device__ static B zero_bnd() { return 0u; }
@@ -246,10 +252,9 @@ __device__ __forceinline__ void row_store(const RowFetch<A>& f, int chunk, int l
     const int i0 = x0 & (kRing - 1), i1 = x1 & (kRing - 1);
     ring_code[i0] = (uint16_t)(c0 * rowstride);
     ring_code[i1] = (uint16_t)(c1 * rowstride);
-    if (i0 < kChunk) { // the copy after the ring's end
-        ring_code[i0 + kRing] = (uint16_t)(c0 * rowstride);
-        ring_code[i1 + kRing] = (uint16_t)(c1 * rowstride);
-    }
+    // the copy after the ring's end
+    if (i0 < kRingTail) ring_code[i0 + kRing] = (uint16_t)(c0 * rowstride);
+    if (i1 < kRingTail) ring_code[i1 + kRing] = (uint16_t)(c1 * rowstride);
     ring_bnd[x0 & (kRing - 1)] = f.d0;
     ring_bnd[x1 & (kRing - 1)] = f.d1;
 }
@@ -309,6 +314,12 @@ __device__ __forceinline__ typename A::V tree_max(const typename A::V (&h)[K])
         for (int k = 0; k + w < K; k += 3 * w)
             t[k] = (k + 2 * w < K) ? A::vmax3(t[k], t[k + w], t[k + 2 * w]) : A::vmax(t[k], t[k + w]);
     return t[0];
+}
+
+// Bits 0 .. n-1 set (n clamped to [0, 16]): the columns of a block inside the pair.
+__device__ __forceinline__ uint32_t cols_mask(int n)
+{
+    return n <= 0 ? 0u : n >= 16 ? 0xFFFFu : (1u << n) - 1u;
 }
 
 struct PassResult {
@@ -421,16 +432,25 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             // so the compiler will not unroll a runtime trip count); a step past
             // steps_max only meets padding rows
             const int s_end = min(s0 + kChunk, steps_max);
-            auto step = [&](const int s, const uint32_t off_lo, const uint32_t off_hi) {
+            // profile words and strip boundary of one step, requested a step ahead
+            struct StepIn {
+                uint32_t plo[KS / 2], phi[KS / 2];
+                Bd bnd;
+            };
+            auto fetch = [&](StepIn& in, const int s, const uint32_t off_lo, const uint32_t off_hi) {
+                prof_load<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
+                prof_load<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
+                in.bnd = ring_bnd[s & (kRing - 1)]; // zero unless a strip precedes
+            };
+            auto step = [&](const int s, const StepIn& in) {
+                const uint32_t* plo = in.plo;
+                const uint32_t* phi = in.phi;
+                const Bd bnd = in.bnd;
                 // left inputs: neighbour's outputs of the previous step; virtual lane 0
                 // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
                 V xH = A::dpp(Hout), xE = A::dpp(Eout);
-                const Bd bnd = ring_bnd[s & (kRing - 1)]; // zero unless a strip precedes
                 xH = A::fix_h(xH, bnd, lfix);
                 xE = A::fix_e(xE, bnd, lfix);
-                uint32_t plo[KS / 2], phi[KS / 2];
-                prof_load<K>((const uint8_t*)prof + off_lo, v_lo, plo);
-                prof_load<K>((const uint8_t*)prof + off_hi, v_hi, phi);
 
                 V E = xE, Hd = Hdiag0;
 #pragma unroll
@@ -467,10 +487,22 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                         const int row_lo = s - v_lo, row_hi = s - v_hi;
                         // first (smallest) column of this row that reaches the target
                         int k_lo = K, k_hi = K;
+                        if constexpr (!A::kWide && K <= 16) {
+                            // bit k of each half: column k equals the target
+                            uint32_t acc = 0;
 #pragma unroll
-                        for (int k = K - 1; k >= 0; --k) {
-                            if (A::eq_lo(Hcol[k], TGT) && strip_base + v_lo * K + k < t.nc) k_lo = k;
-                            if (A::eq_hi(Hcol[k], TGT) && strip_base + v_hi * K + k < t.nc) k_hi = k;
+                            for (int k = K - 1; k >= 0; --k)
+                                acc = raw(as_u(acc) * (u16x2){2, 2} + as_u(A::eq_bits(Hcol[k], TGT)));
+                            const uint32_t ml = (acc & 0xFFFFu) & cols_mask(t.nc - (strip_base + v_lo * K));
+                            const uint32_t mh = (acc >> 16) & cols_mask(t.nc - (strip_base + v_hi * K));
+                            k_lo = ml ? __builtin_ctz(ml) : K;
+                            k_hi = mh ? __builtin_ctz(mh) : K;
+                        } else {
+#pragma unroll
+                            for (int k = K - 1; k >= 0; --k) {
+                                if (A::eq_lo(Hcol[k], TGT) && strip_base + v_lo * K + k < t.nc) k_lo = k;
+                                if (A::eq_hi(Hcol[k], TGT) && strip_base + v_hi * K + k < t.nc) k_hi = k;
+                            }
                         }
                         if (k_lo < K && row_lo >= 0 && row_lo < t.nr && strip_base + v_lo * K + k_lo < rt_lo) {
                             rt_lo = strip_base + v_lo * K + k_lo;
@@ -484,11 +516,17 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 }
             };
             // this chunk's row codes of each virtual lane are contiguous in the ring
+            // (software pipeline: the LDS reads of step s + 1 are in flight while
+            // step s computes)
             const uint16_t* rc_lo = ring_code + ((s0 - v_lo) & (kRing - 1));
             const uint16_t* rc_hi = ring_code + ((s0 - v_hi) & (kRing - 1));
+            StepIn ia, ib;
+            fetch(ia, s0, rc_lo[0], rc_hi[0]);
             for (int s = s0; s < s_end; s += 2) {
-                step(s, rc_lo[0], rc_hi[0]);
-                step(s + 1, rc_lo[1], rc_hi[1]);
+                fetch(ib, s + 1, rc_lo[1], rc_hi[1]);
+                step(s, ia);
+                fetch(ia, s + 2, rc_lo[2], rc_hi[2]);
+                step(s + 1, ib);
                 rc_lo += 2;
                 rc_hi += 2;
             }
