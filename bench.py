@@ -28,6 +28,12 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "gpu-bsw_amd"))
 
+# Hardware queues per process, read once when HIP initialises (before any GPU
+# call here).  The serving loop runs four slots of four streams; with HIP's
+# default of four queues their kernels serialise behind each other's long
+# buckets (measured: p50 latency at 25 % load 26 ms with 4 queues, 7 ms with 16).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import numpy as np  # noqa: E402
 
 import gbsw  # noqa: E402

@@ -127,6 +127,10 @@ int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value)
         if (value != 0 && value != 1) return fail(GBSW_E_ARGUMENT, "f16 option is 0 or 1");
         ctx->ex.f16 = value != 0;
         return GBSW_OK;
+    case GBSW_OPT_GEOMETRY:
+        if (value != kModelClassic && value != kModelOccupancy) return fail(GBSW_E_ARGUMENT, "geometry model is 0 or 1");
+        ctx->ex.kmodel = (int)value;
+        return GBSW_OK;
     default:
         return fail(GBSW_E_ARGUMENT, "unknown option");
     }
@@ -198,6 +202,7 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
         job.alpha = A;
     }
     job.lut_host = b->lut;
+    job.kmodel = ctx->ex.kmodel;
     count_buckets(job, b->len_ref.data(), b->len_qry.data());
     size_t sort_bytes = 0;
     int rc = sort_scratch_bytes(n, &sort_bytes);
@@ -223,7 +228,7 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
     if (e == hipSuccess) e = hipMemcpy(b->qry_off.ptr, qo.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(b->out.ptr, 0, sizeof(int32_t) * 5 * np);
     if (e == hipSuccess)
-        e = hipMemcpy(b->bucket_of_len.ptr, bucket_table(max_k_for(job.alpha)).data(), 65536, hipMemcpyHostToDevice);
+        e = hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, job.kmodel).data(), 65536, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         delete b;
         return fail(e == hipErrorOutOfMemory ? GBSW_E_MEMORY : GBSW_E_DEVICE,

@@ -72,6 +72,12 @@ bool default_f16()
     return !(e && std::atoi(e) == 0);
 }
 
+int default_kmodel()
+{
+    const char* e = std::getenv("GBSW_KMODEL");
+    return (e && std::atoi(e) == kModelClassic) ? kModelClassic : kModelOccupancy;
+}
+
 int default_streams()
 {
     const char* e = std::getenv("GBSW_STREAMS");
@@ -142,6 +148,7 @@ int Exec::init(int dev, int ncu)
     num_cu = ncu;
     streams = default_streams();
     f16 = default_f16();
+    kmodel = default_kmodel();
     hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
     if (e == hipSuccess) e = hipEventCreate(&ev_stop);
@@ -195,17 +202,38 @@ int max_k_for(int alpha)
     return best;
 }
 
-int k_for_cols(int cols, int kmax)
+// Share of the VALU issue rate a geometry reaches at its occupancy: the large
+// DNA buckets issue 0.96 VALU instructions per CU per cycle at 4 waves per SIMD,
+// 0.82 at 2.75, 0.78 at 2 and 0.62 at 1.5 (rocprofv3 counters), so
+// min(1, waves per SIMD / 2.5).
+double occupancy_factor(int K, int alpha)
 {
+    static std::mutex mu;
+    static std::map<int, double> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const int key = K * 64 + alpha;
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const int wg_per_cu = kernel_occupancy(with_alphabet(kBatched, alpha), K, lds_for(K, alpha)); // one wave each
+    const double f = wg_per_cu > 0 ? std::min(1.0, wg_per_cu / 4.0 / 2.5) : 1.0;
+    cache.emplace(key, f);
+    return f;
+}
+
+int k_for_cols(int cols, int alpha, int model)
+{
+    const int kmax = max_k_for(alpha);
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
     int best = ks[0];
-    int64_t best_cost = INT64_MAX;
+    double best_cost = 1e300;
     for (int i = 0; i < nk; ++i) {
         const int K = ks[i];
         if (K > kmax) break;
         const int64_t strips = std::max<int64_t>(1, (cols + kVirtPerPair * K - 1) / (kVirtPerPair * K));
-        const int64_t cost = strips * (K + 2) * 4 + (strips - 1);
+        // padded columns and the per-step cost, per strip
+        double cost = (double)(strips * (K + 2) * 4 + (strips - 1));
+        if (model == kModelOccupancy) cost /= occupancy_factor(K, alpha);
         if (cost < best_cost) { best_cost = cost; best = K; }
     }
     return best;
@@ -218,18 +246,23 @@ int bucket_ids()
     return 2 * nk;
 }
 
-const std::vector<uint8_t>& bucket_table(int kmax)
+const std::vector<uint8_t>& bucket_table(int alpha, int model)
 {
     static std::mutex mu;
     static std::map<int, std::vector<uint8_t>> cache;
     std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find(kmax);
+    const int key = alpha * 4 + model;
+    auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
+    // strip counts, hence the choice, only change at multiples of 32 columns:
+    // the lengths (32(j-1), 32j] share the choice made for 32j
+    std::vector<int> kblk(65536 / 32 + 1);
+    for (size_t j = 0; j < kblk.size(); ++j) kblk[j] = k_for_cols((int)(32 * j), alpha, model);
     std::vector<uint8_t> t(65536);
     for (int m = 0; m < 65536; ++m) {
-        const int K = k_for_cols(m, kmax);
+        const int K = kblk[(size_t)((m + 31) / 32)];
         int idx = 0;
         while (idx < nk && ks[idx] != K) ++idx;
         // pairs spanning several strips get buckets of their own: a single-strip
@@ -237,13 +270,12 @@ const std::vector<uint8_t>& bucket_table(int kmax)
         const bool multi = m > kVirtPerPair * K;
         t[(size_t)m] = (uint8_t)(idx + (multi ? nk : 0));
     }
-    return cache.emplace(kmax, std::move(t)).first->second;
+    return cache.emplace(key, std::move(t)).first->second;
 }
 
 void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry)
 {
-    const int kmax = max_k_for(job.alpha);
-    const std::vector<uint8_t>& tab = bucket_table(kmax);
+    const std::vector<uint8_t>& tab = bucket_table(job.alpha, job.kmodel);
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
     const int nb = bucket_ids();
@@ -277,8 +309,8 @@ int batched_grid_cap(const Exec& ex, int K, int alpha)
     // the f16 and int16 kernels of a geometry share its LDS layout; take the
     // smaller occupancy so the bound holds for either
     const int lds = lds_for(K, alpha);
-    int occ = kernel_occupancy(kBatched, K, lds);
-    if (kernel_has(kBatchedF16, K)) occ = std::min(occ, kernel_occupancy(kBatchedF16, K, lds));
+    int occ = kernel_occupancy(with_alphabet(kBatched, alpha), K, lds);
+    if (kernel_has(kBatchedF16, K)) occ = std::min(occ, kernel_occupancy(with_alphabet(kBatchedF16, alpha), K, lds));
     return std::max(1, occ) * ex.num_cu;
 }
 
@@ -355,23 +387,26 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     GBSW_HIP_TRY(hipMemcpyAsync(ex.colmax.ptr, th + 2304, sizeof(int16_t) * colmax.size(), hipMemcpyHostToDevice, st0));
     GBSW_HIP_TRY(ex.redo.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(1, job.n)));
     KernelArgs a = base_args(ex, job, sc);
-    const int kmax = max_k_for(job.alpha);
     GBSW_HIP_TRY(zero_u32(ex.status.as<uint32_t>(), kStatusWords, st0));
     if (mode == GBSW_PLAIN) {
-        // one geometry for the whole batch, sized by its longest shorter sequence
-        const int K = k_for_cols(job.max_cols, kmax);
+        // one geometry for the whole batch, sized by its longest shorter sequence;
+        // with profiles (DNA) the occupancy model's narrower strips cost the long
+        // pairs more than the extra waves gain (793 vs 593 GCUPS on the 1M-pair
+        // DNA mix), so it applies to profile-free alphabets only
+        const int K = k_for_cols(job.max_cols, job.alpha, profile_free(job.alpha) ? job.kmodel : kModelClassic);
         const int lds = lds_for(K, job.alpha);
         const int64_t nquads = (job.n + kPairsPerWave - 1) / kPairsPerWave;
         a.order = nullptr;
         a.order_n = job.n;
-        const int gf = grid_for(ex, kPlainForward, K, lds, nquads);
-        const int gr = grid_for(ex, kPlainReverse, K, lds, nquads);
+        const int kf = with_alphabet(kPlainForward, job.alpha), kr = with_alphabet(kPlainReverse, job.alpha);
+        const int gf = grid_for(ex, kf, K, lds, nquads);
+        const int gr = grid_for(ex, kr, K, lds, nquads);
         GBSW_HIP_TRY(ex.bnd.reserve((size_t)std::max(gf, gr) * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t)));
-        rc = launch_timed(ex, st0, kPlainForward, K, gf, lds, a, ex.bnd.ptr);
+        rc = launch_timed(ex, st0, kf, K, gf, lds, a, ex.bnd.ptr);
         if (rc) return rc;
         ex.launches.back().pairs = job.n;
         ex.launches.back().cells = job.cells;
-        rc = launch_timed(ex, st0, kPlainReverse, K, gr, lds, a, ex.bnd.ptr);
+        rc = launch_timed(ex, st0, kr, K, gr, lds, a, ex.bnd.ptr);
         if (rc) return rc;
         ex.launches.back().pairs = job.n;
     } else {
@@ -392,7 +427,7 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         for (size_t i = 0; i < nb; ++i) {
             order[i] = i;
             const Bucket& bk = job.buckets[i];
-            kinds[i] = batched_kind(bk, max_sub, ex.f16);
+            kinds[i] = with_alphabet(batched_kind(bk, max_sub, ex.f16), job.alpha);
             ldss[i] = lds_for(bk.K, job.alpha);
             grids[i] = grid_for(ex, kinds[i], bk.K, ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
             bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
@@ -410,7 +445,7 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
             a.order = job.order + bk.begin;
             a.order_n = bk.count;
             a.queue = ex.queue.as<uint32_t>() + i;
-            a.sat_limit = kinds[i] == kBatchedF16 ? kF16Exact - max_sub : 32767 - max_sub;
+            a.sat_limit = (kinds[i] & ~kProfileFree) == kBatchedF16 ? kF16Exact - max_sub : 32767 - max_sub;
             hipStream_t st = (j % ns) == 0 ? st0 : ex.aux[j % ns - 1];
             rc = launch_timed(ex, st, kinds[i], bk.K, grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
             if (rc) return rc;
@@ -449,9 +484,10 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
         a.sat_limit = INT_MAX;
         a.queue = ex.queue.as<uint32_t>();
         GBSW_HIP_TRY(zero_u32(a.queue, 1, ex.stream));
-        const int grid = grid_for(ex, kBatchedWide, K, lds, (count + kPairsPerWave - 1) / kPairsPerWave);
+        const int kw = with_alphabet(kBatchedWide, job.alpha);
+        const int grid = grid_for(ex, kw, K, lds, (count + kPairsPerWave - 1) / kPairsPerWave);
         GBSW_HIP_TRY(ex.bnd.reserve((size_t)grid * kPairsPerWave * (size_t)a.bnd_rows * 8));
-        int rc = launch_timed(ex, ex.stream, kBatchedWide, K, grid, lds, a, ex.bnd.ptr);
+        int rc = launch_timed(ex, ex.stream, kw, K, grid, lds, a, ex.bnd.ptr);
         if (rc) return rc;
         ex.launches.back().pairs = count;
         GBSW_HIP_TRY(hipMemcpyAsync(ex.status_host.ptr, ex.status.ptr, sizeof(int32_t) * kStatusWords,

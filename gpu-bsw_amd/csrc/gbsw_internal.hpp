@@ -79,6 +79,7 @@ struct Exec {
     int32_t max_sub = 0;
     int streams = kAuxStreams + 1; // streams for the length buckets (1 = serialised)
     bool f16 = true;               // exact f16 kernels where the score bound allows
+    int kmodel = 1;                // length -> geometry model of new batches (kModel*)
     int init(int device, int num_cu);
     void destroy();
 };
@@ -93,6 +94,7 @@ struct Job {
     const uint64_t* qry_off = nullptr;
     int32_t* out = nullptr;          // 5*n int32
     int32_t alpha = 0;               // residue codes
+    int kmodel = 1;                  // length -> geometry model (kModel*), fixed at creation
     const uint8_t* lut_host = nullptr; // 256 entries
     int32_t max_rows = 0, max_cols = 0;
     int64_t cells = 0;
@@ -108,13 +110,17 @@ struct Job {
 
 // Largest compiled K that fits a 64 KiB workgroup for this alphabet size.
 int max_k_for(int alpha);
-// K for a shorter sequence of `cols` residues (strips x (K + 2) minimised).
-int k_for_cols(int cols, int kmax);
+// Length -> geometry models: strips x (K + 2) minimised, or that cost divided by
+// the VALU issue share the geometry's occupancy allows.
+constexpr int kModelClassic = 0;
+constexpr int kModelOccupancy = 1;
+// K for a shorter sequence of `cols` residues.
+int k_for_cols(int cols, int alpha, int model);
 // Bucket ids: position of K in kernel_k_list, plus the list's length for pairs
 // that span several strips (bucket_ids() in all).
 int bucket_ids();
-// Host table shorter-length -> bucket id, 65536 entries, cached per kmax.
-const std::vector<uint8_t>& bucket_table(int kmax);
+// Host table shorter-length -> bucket id, 65536 entries, cached per (alpha, model).
+const std::vector<uint8_t>& bucket_table(int alpha, int model);
 // Key bits of the device radix sort: bucket id << 16 | (65535 - longer length).
 constexpr int kSortBits = 21;
 // Fill job.buckets from per-pair lengths (host), ordered by ascending K.

@@ -127,6 +127,7 @@ struct Packed16 {
         return raw(one - __builtin_elementwise_min(as_u(x ^ t), one));
     }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
+    __device__ static V sval(uint32_t packed) { return packed; } // two table entries, low / high half
     __device__ static B load_bnd(const B* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
     __device__ static B zero_bnd() { return 0u; }
 };
@@ -202,6 +203,10 @@ struct Wide32 {
     __device__ static bool eq_lo(V x, V t) { return x.l == t.l; }
     __device__ static bool eq_hi(V x, V t) { return x.h == t.h; }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
+    __device__ static V sval(uint32_t packed)
+    {
+        return V{(int32_t)(int16_t)(packed & 0xFFFFu), (int32_t)(int16_t)(packed >> 16)};
+    }
     __device__ static V ltmask(V a, V b) { return V{a.l < b.l ? -1 : 0, a.h < b.h ? -1 : 0}; }
     __device__ static V sel(V m, V a, V b)
     {
@@ -332,7 +337,10 @@ struct PassResult {
 //
 // Forward key: (score << 40) | ((2^20-1-row) << 20) | (2^20-1-prio(col)), max wins.
 // Reverse key: ((2^20-1-col) << 20) | (2^20-1-row), max wins (target = score).
-template <class A, int K, bool REV>
+// PF (profile-free): substitution scores come from the (alpha+1)^2 table in the
+// LDS header, addressed by the row code from the ring and per-lane column codes
+// held in registers for the strip; otherwise from the per-strip profile.
+template <class A, int K, bool REV, bool PF>
 __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& t, int32_t target,
                                typename A::B* bnd_g)
 {
@@ -344,7 +352,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     const int lane = threadIdx.x;
     const int g = lane >> 4, l = lane & 15;
     const int alpha = a.alpha;
-    const int rowstride = WA * 2;
+    const int rowstride = PF ? (a.alpha + 1) * 2 : WA * 2; // bytes between rows of the score source
     const LaneFix lfix = lane_fix(l);
     uint8_t* gb = sm.group(g);
     int16_t* prof = (int16_t*)gb;
@@ -383,19 +391,34 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         const bool use_bnd = strip > 0;
         const bool write_bnd = live && strip + 1 < strips_g;
 
-        // ---- profile of this strip: prof[a*W + j] = s(code a, column strip_base+j)
+        // ---- profile of this strip: prof[a*W + j] = s(code a, column strip_base+j);
+        // profile-free: byte offsets of the lane's column codes in a table row
+        // (low half: virtual lane l, high half: l + 16; the padding code alpha for
+        // columns outside the pair)
+        uint32_t coff[PF ? K : 1];
         __syncthreads();
-        for (int j = l; j < WA; j += 16) {
-            int vj, kj;
-            prof_slot<K>(j, vj, kj);
-            const int c = strip_base + vj * K + kj;
-            int code = -1;
-            if (live && kj < K && c < t.nc) {
-                code = lut[t.cols[t.rev ? t.nc_full - 1 - c : c]];
-                if (code >= alpha) code = -1;
+        if constexpr (!PF) {
+            for (int j = l; j < WA; j += 16) {
+                int vj, kj;
+                prof_slot<K>(j, vj, kj);
+                const int c = strip_base + vj * K + kj;
+                int code = -1;
+                if (live && kj < K && c < t.nc) {
+                    code = lut[t.cols[t.rev ? t.nc_full - 1 - c : c]];
+                    if (code >= alpha) code = -1;
+                }
+                for (int r = 0; r <= alpha; ++r)
+                    prof[r * WA + j] = A::prof_val((code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg);
             }
-            for (int r = 0; r <= alpha; ++r)
-                prof[r * WA + j] = A::prof_val((code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int c_lo = strip_base + v_lo * K + k, c_hi = strip_base + v_hi * K + k;
+                int code_lo = alpha, code_hi = alpha;
+                if (live && c_lo < t.nc) code_lo = min((int)lut[t.cols[t.rev ? t.nc_full - 1 - c_lo : c_lo]], alpha);
+                if (live && c_hi < t.nc) code_hi = min((int)lut[t.cols[t.rev ? t.nc_full - 1 - c_hi : c_hi]], alpha);
+                coff[k] = (uint32_t)(code_lo * 2) | ((uint32_t)(code_hi * 2) << 16);
+            }
         }
         // ---- row ring: rows [-32, 0) padding, [0, 32) now, [32, 64) in flight
         {
@@ -432,14 +455,26 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             // so the compiler will not unroll a runtime trip count); a step past
             // steps_max only meets padding rows
             const int s_end = min(s0 + kChunk, steps_max);
-            // profile words and strip boundary of one step, requested a step ahead
+            // profile words (or the K packed scores) and strip boundary of one step,
+            // requested a step ahead
             struct StepIn {
-                uint32_t plo[KS / 2], phi[KS / 2];
+                uint32_t plo[PF ? 1 : KS / 2], phi[PF ? 1 : KS / 2];
+                uint32_t sv[PF ? K : 1];
                 Bd bnd;
             };
             auto fetch = [&](StepIn& in, const int s, const uint32_t off_lo, const uint32_t off_hi) {
-                prof_load<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
-                prof_load<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
+                if constexpr (!PF) {
+                    prof_load<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
+                    prof_load<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
+                } else {
+                    const uint8_t* tab = (const uint8_t*)sub;
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const uint32_t s_lo = *(const uint16_t*)(tab + off_lo + (coff[k] & 0xFFFFu));
+                        const uint32_t s_hi = *(const uint16_t*)(tab + off_hi + (coff[k] >> 16));
+                        in.sv[k] = s_lo | (s_hi << 16);
+                    }
+                }
                 in.bnd = ring_bnd[s & (kRing - 1)]; // zero unless a strip precedes
             };
             auto step = [&](const int s, const StepIn& in) {
@@ -455,7 +490,9 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 V E = xE, Hd = Hdiag0;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const V S = A::template prof<K>(plo, phi, k);
+                    V S;
+                    if constexpr (PF) S = A::sval(in.sv[k]);
+                    else S = A::template prof<K>(plo, phi, k);
                     const V D = A::add(Hd, S);
                     Hd = Hcol[k];
                     const V H = A::cell(D, E, Fcol[k]);
@@ -721,10 +758,23 @@ __device__ __forceinline__ void reverse_result(const KernelArgs& a, int64_t p, c
     else { *ref_begin = orow; *query_begin = ocol; }                       // 596-600
 }
 
+// Byte->code table, substitution scores and column maxima into LDS.  With PF the
+// scores form an (alpha+1)^2 table in the policy's representation whose last row
+// and column (the padding code) hold -16384; otherwise the raw alpha^2 scores the
+// profile build reads.
+template <class A, bool PF>
 __device__ __forceinline__ void stage_tables(const KernelArgs& a, const Smem& sm)
 {
     for (int i = threadIdx.x; i < 256; i += kWave) sm.lut()[i] = a.lut[i];
-    for (int i = threadIdx.x; i < a.alpha * a.alpha; i += kWave) sm.sub()[i] = a.sub[i];
+    if constexpr (PF) {
+        const int A1 = a.alpha + 1;
+        for (int i = threadIdx.x; i < A1 * A1; i += kWave) {
+            const int r = i / A1, c = i % A1;
+            sm.sub()[i] = A::prof_val((r < a.alpha && c < a.alpha) ? a.sub[r * a.alpha + c] : kNeg);
+        }
+    } else {
+        for (int i = threadIdx.x; i < a.alpha * a.alpha; i += kWave) sm.sub()[i] = a.sub[i];
+    }
     for (int i = threadIdx.x; i < a.alpha; i += kWave) sm.colmax()[i] = a.colmax[i];
     __syncthreads();
 }
@@ -752,19 +802,19 @@ __device__ __forceinline__ typename A::B* group_bnd(const KernelArgs& a)
 
 // Plain form, forward pass: quads of consecutive pairs in input order, statically
 // strided over the grid (one launch per batch, like driver.cpp:176-185).
-template <int K>
+template <int K, bool PF>
 __global__ __launch_bounds__(64) void k_plain_forward(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const Smem sm = make_smem<Packed16, K>(smem_raw, a);
-    stage_tables(a, sm);
+    stage_tables<Packed16, PF>(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     Packed16::B* bnd_g = group_bnd<Packed16>(a);
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
     for (int64_t q = blockIdx.x; q < nquads; q += gridDim.x) {
         const int64_t p = slot_pair(a, q * kPairsPerWave + g);
         const Task t = forward_task(a, p);
-        const PassResult r = run_pass<Packed16, K, false>(a, sm, t, 0, bnd_g);
+        const PassResult r = run_pass<Packed16, K, false, PF>(a, sm, t, 0, bnd_g);
         const FwdOut f = forward_result(a, p, r.key);
         if (p >= 0 && l == 0) {
             a.out[kOutScore * a.n + p] = f.score;
@@ -776,12 +826,12 @@ __global__ __launch_bounds__(64) void k_plain_forward(KernelArgs a)
 
 // Plain form, reverse pass: reads the forward results of the same buffer
 // (driver.cpp:199-208, with no host round trip for the block size).
-template <int K>
+template <int K, bool PF>
 __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const Smem sm = make_smem<Packed16, K>(smem_raw, a);
-    stage_tables(a, sm);
+    stage_tables<Packed16, PF>(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     Packed16::B* bnd_g = group_bnd<Packed16>(a);
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
@@ -794,7 +844,7 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
             f.query_end = a.out[kOutQueryEnd * a.n + p];
         }
         const Task t = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult r = run_pass<Packed16, K, true>(a, sm, t, f.score, bnd_g);
+        const PassResult r = run_pass<Packed16, K, true, PF>(a, sm, t, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, t, r.key, &rb, &qb);
         if (p >= 0 && l == 0 && f.score >= 0) {
@@ -807,12 +857,12 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
 // Batched form: persistent workgroups pull quads of length-sorted pairs from a
 // device work queue and run forward then reverse back to back.  With A = Wide32
 // it recomputes the pairs the int16 kernels queued as possibly saturated.
-template <class A, int K>
+template <class A, int K, bool PF>
 __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
     const Smem sm = make_smem<A, K>(smem_raw, a);
-    stage_tables(a, sm);
+    stage_tables<A, PF>(a, sm);
     const int l = threadIdx.x & 15;
     const int g = threadIdx.x >> 4;
     typename A::B* bnd_g = group_bnd<A>(a);
@@ -824,10 +874,10 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
         if ((int64_t)q >= nquads) break;
         const int64_t p = slot_pair(a, (int64_t)q * kPairsPerWave + g);
         const Task tf = forward_task(a, p);
-        const PassResult rf = run_pass<A, K, false>(a, sm, tf, 0, bnd_g);
+        const PassResult rf = run_pass<A, K, false, PF>(a, sm, tf, 0, bnd_g);
         const FwdOut f = forward_result(a, p, rf.key);
         const Task tr = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult rr = run_pass<A, K, true>(a, sm, tr, f.score, bnd_g);
+        const PassResult rr = run_pass<A, K, true, PF>(a, sm, tr, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, tr, rr.key, &rb, &qb);
         if (p >= 0 && l == 0 && f.score >= 0) {
@@ -848,27 +898,27 @@ __global__ void k_zero_u32(uint32_t* p, int n)
 
 // ---------------------------------------------------------------- host side
 
-template <int K>
+template <int K, bool PF>
 static const void* kernel_ptr(int kind)
 {
     switch (kind) {
-    case kPlainForward: return (const void*)&k_plain_forward<K>;
-    case kPlainReverse: return (const void*)&k_plain_reverse<K>;
-    case kBatched: return (const void*)&k_batched<Packed16, K>;
+    case kPlainForward: return (const void*)&k_plain_forward<K, PF>;
+    case kPlainReverse: return (const void*)&k_plain_reverse<K, PF>;
+    case kBatched: return (const void*)&k_batched<Packed16, K, PF>;
     default: return nullptr;
     }
 }
 
-template <int K>
+template <int K, bool PF>
 static const void* wide_ptr()
 {
-    return (const void*)&k_batched<Wide32, K>;
+    return (const void*)&k_batched<Wide32, K, PF>;
 }
 
-template <int K>
+template <int K, bool PF>
 static const void* f16_ptr()
 {
-    return (const void*)&k_batched<PackedF16, K>;
+    return (const void*)&k_batched<PackedF16, K, PF>;
 }
 
 #define GBSW_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
@@ -879,11 +929,12 @@ static const void* f16_ptr()
 // scores below 2048
 #define GBSW_F16_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20)
 
-static const void* kernel_for(int kind, int K)
+template <bool PF>
+static const void* kernel_for_pf(int kind, int K)
 {
     if (kind == kBatchedWide) {
         switch (K) {
-#define GBSW_CASE(k) case k: return wide_ptr<k>();
+#define GBSW_CASE(k) case k: return wide_ptr<k, PF>();
             GBSW_WIDE_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
         default: return nullptr;
@@ -891,18 +942,24 @@ static const void* kernel_for(int kind, int K)
     }
     if (kind == kBatchedF16) {
         switch (K) {
-#define GBSW_CASE(k) case k: return f16_ptr<k>();
+#define GBSW_CASE(k) case k: return f16_ptr<k, PF>();
             GBSW_F16_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
         default: return nullptr;
         }
     }
     switch (K) {
-#define GBSW_CASE(k) case k: return kernel_ptr<k>(kind);
+#define GBSW_CASE(k) case k: return kernel_ptr<k, PF>(kind);
         GBSW_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
     default: return nullptr;
     }
+}
+
+// kind may carry kProfileFree (the instantiation for large alphabets)
+static const void* kernel_for(int kind, int K)
+{
+    return (kind & kProfileFree) ? kernel_for_pf<true>(kind & ~kProfileFree, K) : kernel_for_pf<false>(kind, K);
 }
 
 bool kernel_supported(int K) { return kernel_for(kPlainForward, K) != nullptr; }

@@ -279,7 +279,7 @@ int gbsw_server::init_tables(int which)
         for (int c = 0; c < 256; ++c) accept[c] = c < 91;
     }
     t.kmax = max_k_for(t.alpha);
-    t.bucket_of_len = bucket_table(t.kmax);
+    t.bucket_of_len = bucket_table(t.alpha, slots[0]->ex.kmodel);
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
     const int nb = bucket_ids();
@@ -432,6 +432,7 @@ void gbsw_server::admit(Slot& s)
     j.alphabet = T.alphabet;
     j.n = pairs;
     j.alpha = T.alpha;
+    j.kmodel = s.ex.kmodel;
     j.lut_host = T.lut;
     j.max_rows = max_rows;
     j.max_cols = max_cols;
@@ -557,7 +558,7 @@ void gbsw_server::finish(Slot& s, bool device_ok)
         double ms = 0.0;
         rc = complete_job(s.ex, s.job, &s.sc, &ms);
         bool redone = false;
-        for (const LaunchRec& L : s.ex.launches) redone |= L.kind == kBatchedWide;
+        for (const LaunchRec& L : s.ex.launches) redone |= (L.kind & ~kProfileFree) == kBatchedWide;
         if (rc == GBSW_OK && redone) {
             // int32 recompute ran after the copy-back: copy the results again
             hipError_t e = hipMemcpyAsync(s.out_host.ptr, s.out.ptr, sizeof(int32_t) * 5 * (size_t)s.job.n,

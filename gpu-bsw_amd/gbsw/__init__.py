@@ -112,7 +112,7 @@ EXPORTED = (
     "gbsw_server_wait", "gbsw_server_poll", "gbsw_server_stats", "gbsw_host_alloc", "gbsw_host_free",
     "gbsw_context_set_option",
 )
-OPT_STREAMS, OPT_F16 = 1, 2
+OPT_STREAMS, OPT_F16, OPT_GEOMETRY = 1, 2, 3
 
 _lib = None
 
@@ -216,7 +216,7 @@ class Context:
             pass
 
     def set_option(self, option, value):
-        """OPT_STREAMS (1..4) or OPT_F16 (0/1); see include/gbsw.h."""
+        """OPT_STREAMS (1..4), OPT_F16 (0/1) or OPT_GEOMETRY (0/1); see include/gbsw.h."""
         _check(self._lib.gbsw_context_set_option(self._h, option, int(value)))
 
     def batch(self, alphabet, ref, ref_off, query, query_off):

@@ -82,6 +82,14 @@ int gbsw_context_destroy(gbsw_context* ctx);
  *          0: every bucket runs the int16 kernel.  Results are identical. */
 #define GBSW_OPT_STREAMS 1
 #define GBSW_OPT_F16 2
+/* GEOMETRY: how a pair's shorter length picks its kernel geometry (K columns per
+ *          lane), for batches created afterwards.  0: fewest padded columns and
+ *          steps, strips x (K + 2); 1 (default; GBSW_KMODEL=0 selects 0): that
+ *          cost divided by the share of the VALU issue rate the geometry's
+ *          occupancy allows.  Applies to the length buckets of
+ *          gbsw_batch_align(GBSW_BATCHED), and to the plain form for protein
+ *          (the plain form uses 0 for DNA).  Results are identical. */
+#define GBSW_OPT_GEOMETRY 3
 int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value);
 
 /* Message of the last error on this thread ("" if none). */
@@ -112,7 +120,10 @@ int gbsw_batch_stats(const gbsw_batch* batch, gbsw_stats* out);
 
 /* One kernel launch of the last gbsw_batch_align (index 0 .. stats.launches-1). */
 typedef struct {
-    int32_t kind;             /* 0 plain forward, 1 plain reverse, 2 batched fwd+rev  */
+    int32_t kind;             /* 0 plain forward, 1 plain reverse, 2 batched fwd+rev
+                                 int16, 3 batched int32 (saturated pairs), 4 batched
+                                 f16; +256: scores from the substitution table
+                                 (alphabets of 8+ codes) instead of a profile       */
     int32_t columns_per_lane; /* K: columns per virtual lane (strip = 32*K columns)  */
     int32_t workgroups;       /* grid size (64-lane workgroups)                       */
     int64_t pairs;            /* pairs processed by the launch                        */
