@@ -86,9 +86,9 @@ int default_streams()
 }
 
 // LDS bytes of a workgroup; bnd_bytes = 4 (packed int16) or 8 (int32 kernel).
-int lds_for(int K, int alpha, int bnd_bytes = 4)
+int lds_for(int K, int alpha, bool p8, int bnd_bytes = 4)
 {
-    return lds_layout(kVirtPerPair * kstride(K), alpha, bnd_bytes).total;
+    return lds_layout(prof_row_bytes(K, p8), alpha, bnd_bytes).total;
 }
 
 int grid_for(const Exec& ex, int kind, int K, int lds, int64_t nquads)
@@ -192,13 +192,13 @@ void Exec::destroy()
     ev_start = ev_stop = ev_fork = nullptr;
 }
 
-int max_k_for(int alpha)
+int max_k_for(int alpha, bool p8)
 {
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
     int best = ks[0];
     for (int i = 0; i < nk; ++i)
-        if (lds_for(ks[i], alpha) <= 65536) best = ks[i];
+        if (lds_for(ks[i], alpha, p8) <= 65536) best = ks[i];
     return best;
 }
 
@@ -214,7 +214,8 @@ double occupancy_factor(int K, int alpha)
     const int key = K * 64 + alpha;
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    const int wg_per_cu = kernel_occupancy(with_alphabet(kBatched, alpha), K, lds_for(K, alpha)); // one wave each
+    const bool p8 = byte_profile_alphabet(alpha);
+    const int wg_per_cu = kernel_occupancy(with_profile(kBatched, p8), K, lds_for(K, alpha, p8)); // one wave each
     const double f = wg_per_cu > 0 ? std::min(1.0, wg_per_cu / 4.0 / 2.5) : 1.0;
     cache.emplace(key, f);
     return f;
@@ -222,7 +223,7 @@ double occupancy_factor(int K, int alpha)
 
 int k_for_cols(int cols, int alpha, int model)
 {
-    const int kmax = max_k_for(alpha);
+    const int kmax = max_k_for(alpha, byte_profile_alphabet(alpha));
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
     int best = ks[0];
@@ -308,9 +309,10 @@ int batched_grid_cap(const Exec& ex, int K, int alpha)
 {
     // the f16 and int16 kernels of a geometry share its LDS layout; take the
     // smaller occupancy so the bound holds for either
-    const int lds = lds_for(K, alpha);
-    int occ = kernel_occupancy(with_alphabet(kBatched, alpha), K, lds);
-    if (kernel_has(kBatchedF16, K)) occ = std::min(occ, kernel_occupancy(with_alphabet(kBatchedF16, alpha), K, lds));
+    const bool p8 = byte_profile_alphabet(alpha);
+    const int lds = lds_for(K, alpha, p8);
+    int occ = kernel_occupancy(with_profile(kBatched, p8), K, lds);
+    if (!p8 && kernel_has(kBatchedF16, K)) occ = std::min(occ, kernel_occupancy(kBatchedF16, K, lds));
     return std::max(1, occ) * ex.num_cu;
 }
 
@@ -332,7 +334,7 @@ int sort_scratch_bytes(int64_t n, size_t* bytes)
     return GBSW_OK;
 }
 
-int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub)
+int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8)
 {
     if (sc->alphabet != job.alphabet) return fail(GBSW_E_ARGUMENT, "scoring alphabet differs from the batch alphabet");
     if (sc->gap_open > 0 || sc->gap_extend > 0) return fail(GBSW_E_SCORING, "gap_open and gap_extend must be <= 0");
@@ -352,6 +354,9 @@ int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>
     for (int16_t v : sub)
         if (v < -16000 || v > 16000) return fail(GBSW_E_SCORING, "substitution scores must lie in [-16000, 16000]");
     *max_sub = std::max(0, mx);
+    int lo = 0;
+    for (int16_t v : sub) lo = std::min<int>(lo, v);
+    *p8 = byte_profile_alphabet(A) && lo >= -127 && mx <= 127;
     // result keys hold scores in 23 bits; int16 saturation is handled per pair
     if ((int64_t)job.max_cols * *max_sub >= (1 << 23)) return fail(GBSW_E_RANGE, "possible score exceeds 2^23");
     return GBSW_OK;
@@ -362,9 +367,14 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     ex.launches.clear();
     std::vector<int16_t> sub;
     int max_sub = 0;
-    int rc = prepare_scoring(job, sc, sub, &max_sub);
+    bool p8 = false;
+    int rc = prepare_scoring(job, sc, sub, &max_sub, &p8);
     if (rc) return rc;
     ex.max_sub = max_sub;
+    ex.p8 = p8;
+    // geometries were chosen for the byte profile of a large alphabet; scores
+    // outside int8 take the int16 profile, in the widest geometry that fits
+    const int kclamp = (byte_profile_alphabet(job.alpha) && !p8) ? max_k_for(job.alpha, false) : 1 << 30;
     hipStream_t st0 = ex.stream;
     GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st0));
     if (job.n == 0) {
@@ -393,12 +403,13 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         // with profiles (DNA) the occupancy model's narrower strips cost the long
         // pairs more than the extra waves gain (793 vs 593 GCUPS on the 1M-pair
         // DNA mix), so it applies to profile-free alphabets only
-        const int K = k_for_cols(job.max_cols, job.alpha, profile_free(job.alpha) ? job.kmodel : kModelClassic);
-        const int lds = lds_for(K, job.alpha);
+        const int K = std::min(kclamp, k_for_cols(job.max_cols, job.alpha,
+                                                  byte_profile_alphabet(job.alpha) ? job.kmodel : kModelClassic));
+        const int lds = lds_for(K, job.alpha, p8);
         const int64_t nquads = (job.n + kPairsPerWave - 1) / kPairsPerWave;
         a.order = nullptr;
         a.order_n = job.n;
-        const int kf = with_alphabet(kPlainForward, job.alpha), kr = with_alphabet(kPlainReverse, job.alpha);
+        const int kf = with_profile(kPlainForward, p8), kr = with_profile(kPlainReverse, p8);
         const int gf = grid_for(ex, kf, K, lds, nquads);
         const int gr = grid_for(ex, kr, K, lds, nquads);
         GBSW_HIP_TRY(ex.bnd.reserve((size_t)std::max(gf, gr) * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t)));
@@ -422,14 +433,15 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         // work item takes longest (largest K) first, so a bucket with few long
         // pairs overlaps the bulk instead of running alone at the end.
         std::vector<size_t> order(nb);
-        std::vector<int> grids(nb), ldss(nb), kinds(nb);
+        std::vector<int> grids(nb), ldss(nb), kinds(nb), kk(nb);
         size_t bnd_words = 0;
         for (size_t i = 0; i < nb; ++i) {
             order[i] = i;
             const Bucket& bk = job.buckets[i];
-            kinds[i] = with_alphabet(batched_kind(bk, max_sub, ex.f16), job.alpha);
-            ldss[i] = lds_for(bk.K, job.alpha);
-            grids[i] = grid_for(ex, kinds[i], bk.K, ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
+            kk[i] = std::min(kclamp, bk.K);
+            kinds[i] = with_profile(batched_kind(bk, max_sub, ex.f16 && !p8), p8);
+            ldss[i] = lds_for(kk[i], job.alpha, p8);
+            grids[i] = grid_for(ex, kinds[i], kk[i], ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
             bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
         }
         std::sort(order.begin(), order.end(),
@@ -445,9 +457,9 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
             a.order = job.order + bk.begin;
             a.order_n = bk.count;
             a.queue = ex.queue.as<uint32_t>() + i;
-            a.sat_limit = (kinds[i] & ~kProfileFree) == kBatchedF16 ? kF16Exact - max_sub : 32767 - max_sub;
+            a.sat_limit = (kinds[i] & ~kByteProfile) == kBatchedF16 ? kF16Exact - max_sub : 32767 - max_sub;
             hipStream_t st = (j % ns) == 0 ? st0 : ex.aux[j % ns - 1];
-            rc = launch_timed(ex, st, kinds[i], bk.K, grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
+            rc = launch_timed(ex, st, kinds[i], kk[i], grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
             if (rc) return rc;
             ex.launches.back().pairs = bk.count;
             ex.launches.back().cells = bk.cells;
@@ -476,7 +488,7 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
         int K = wk[nk - 1];
         for (int i = 0; i < nk; ++i)
             if (kVirtPerPair * wk[i] >= job.max_cols) { K = wk[i]; break; }
-        const int lds = lds_for(K, job.alpha, 8);
+        const int lds = lds_for(K, job.alpha, ex.p8, 8);
         const int64_t count = status[kStatusSaturated];
         KernelArgs a = base_args(ex, job, sc);
         a.order = ex.redo.as<int32_t>();
@@ -484,7 +496,7 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
         a.sat_limit = INT_MAX;
         a.queue = ex.queue.as<uint32_t>();
         GBSW_HIP_TRY(zero_u32(a.queue, 1, ex.stream));
-        const int kw = with_alphabet(kBatchedWide, job.alpha);
+        const int kw = with_profile(kBatchedWide, ex.p8);
         const int grid = grid_for(ex, kw, K, lds, (count + kPairsPerWave - 1) / kPairsPerWave);
         GBSW_HIP_TRY(ex.bnd.reserve((size_t)grid * kPairsPerWave * (size_t)a.bnd_rows * 8));
         int rc = launch_timed(ex, ex.stream, kw, K, grid, lds, a, ex.bnd.ptr);

@@ -77,6 +77,7 @@ struct Exec {
     PinnedBuffer tables_host;    // staging of sub / lut / colmax for the async copies
     std::vector<LaunchRec> launches;
     int32_t max_sub = 0;
+    bool p8 = false;               // the current job uses the int8 profile
     int streams = kAuxStreams + 1; // streams for the length buckets (1 = serialised)
     bool f16 = true;               // exact f16 kernels where the score bound allows
     int kmodel = 1;                // length -> geometry model of new batches (kModel*)
@@ -108,8 +109,12 @@ struct Job {
     size_t sort_temp_bytes = 0;
 };
 
-// Largest compiled K that fits a 64 KiB workgroup for this alphabet size.
-int max_k_for(int alpha);
+// Largest compiled K that fits a 64 KiB workgroup for this alphabet size and
+// profile width.
+int max_k_for(int alpha, bool p8);
+// Large alphabets use the int8 profile when the scoring fits it; geometries of a
+// batch are chosen assuming it.
+inline bool byte_profile_alphabet(int alpha) { return alpha >= kByteProfileAlpha; }
 // Length -> geometry models: strips x (K + 2) minimised, or that cost divided by
 // the VALU issue share the geometry's occupancy allows.
 constexpr int kModelClassic = 0;
@@ -134,8 +139,9 @@ int batched_kind(const Bucket& bk, int32_t max_sub, bool allow_f16);
 // Scratch bytes of the radix sort for n pairs.
 int sort_scratch_bytes(int64_t n, size_t* bytes);
 
-// Validate a scoring for a job and build its substitution table.
-int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub);
+// Validate a scoring for a job and build its substitution table; *p8: the job
+// can use the int8 profile.
+int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8);
 
 // Enqueue a whole alignment job (binning if batched, kernels) on ex.stream.
 // Asynchronous: ex.ev_stop is recorded after the last kernel and the status words

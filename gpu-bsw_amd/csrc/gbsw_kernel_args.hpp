@@ -70,24 +70,37 @@ __host__ __device__ constexpr int32_t kremstride(int32_t K)
 // Profile columns reserved per virtual lane.
 __host__ __device__ constexpr int32_t kstride(int32_t K) { return K / 8 * 8 + kremstride(K); }
 
-// Alphabets of at least kProfileFreeAlpha codes (protein) read substitution
-// scores from one (alpha+1)^2 table per step instead of a per-strip profile of
-// (alpha+1) rows per pair, whose LDS would leave room for few waves per CU.
-constexpr int kProfileFreeAlpha = 8;
-__host__ __device__ inline bool profile_free(int32_t alpha) { return alpha >= kProfileFreeAlpha; }
-
-// bnd_bytes: bytes of one strip-boundary entry (4 for packed int16, 8 for int32).
-__host__ __device__ inline LdsLayout lds_layout(int32_t W, int32_t alpha, int32_t bnd_bytes)
+// Byte profile (alphabets of kByteProfileAlpha codes or more, i.e. protein,
+// with every score in [-127, 127]): one int8 per column, -128 for padding, so
+// the (alpha+1)-row profile of a strip takes half the LDS and leaves room for
+// more waves per CU.  Layout of one row: K/16 blocks of 16 columns for all 32
+// virtual lanes (one b128 read each), then the K%16 remaining columns padded to
+// a b32/b64/b128 read (kremstride8).
+constexpr int kByteProfileAlpha = 8;
+__host__ __device__ constexpr int32_t kremstride8(int32_t K)
 {
-    const bool pf = profile_free(alpha);
+    return K % 16 == 0 ? 0 : K % 16 <= 4 ? 4 : K % 16 <= 8 ? 8 : 16;
+}
+__host__ __device__ constexpr int32_t kstride8(int32_t K) { return K / 16 * 16 + kremstride8(K); }
+
+// Bytes of one profile row (one residue code, all columns of a strip).
+__host__ __device__ constexpr int32_t prof_row_bytes(int32_t K, bool bytes8)
+{
+    return bytes8 ? kVirtPerPair * kstride8(K) : kVirtPerPair * kstride(K) * 2;
+}
+
+// row_bytes: prof_row_bytes of the kernel; bnd_bytes: bytes of one
+// strip-boundary entry (4 for packed int16, 8 for int32).
+__host__ __device__ inline LdsLayout lds_layout(int32_t row_bytes, int32_t alpha, int32_t bnd_bytes)
+{
     LdsLayout L;
     L.lut_off = 0;                                       // 256 B
-    L.sub_off = 256;                                     // alpha^2 (or (alpha+1)^2) int16
-    L.colmax_off = L.sub_off + align_up(2 * (pf ? (alpha + 1) * (alpha + 1) : alpha * alpha), 16); // 32 int16
+    L.sub_off = 256;                                     // alpha^2 int16
+    L.colmax_off = L.sub_off + align_up(2 * alpha * alpha, 16); // 32 int16
     L.info_off = L.colmax_off + 2 * 32;                  // 4 groups x 8 int32
     L.group_off = align_up(L.info_off + 4 * 8 * 4, 128);
     L.xch_off = L.group_off;                             // 64 lanes x 2 keys x 8 B
-    L.prof_bytes = pf ? 0 : align_up((alpha + 1) * W * 2, 128);
+    L.prof_bytes = align_up((alpha + 1) * row_bytes, 128);
     L.ring_code_off = L.prof_bytes;
     L.ring_bnd_off = L.ring_code_off + kRingCodes * 2;
     L.group_bytes = align_up(L.ring_bnd_off + kRing * bnd_bytes, 128);

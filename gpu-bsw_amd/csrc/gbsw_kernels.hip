@@ -127,7 +127,21 @@ struct Packed16 {
         return raw(one - __builtin_elementwise_min(as_u(x ^ t), one));
     }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
-    __device__ static V sval(uint32_t packed) { return packed; } // two table entries, low / high half
+    // column k of a byte profile: byte k%4 of word k/4, sign-extended into each
+    // half.  v_perm's selectors 8..11 copy the sign bits of bytes 1, 3, 5, 7, so
+    // odd bytes take one instruction; even ones go to the high byte of each half
+    // and shift down.
+    template <int K>
+    __device__ static V prof8(const uint32_t* plo, const uint32_t* phi, int k)
+    {
+        const uint32_t wl = plo[k / 4], wh = phi[k / 4];
+        switch (k & 3) {
+        case 1: return __builtin_amdgcn_perm(wh, wl, 0x0A050801u);
+        case 3: return __builtin_amdgcn_perm(wh, wl, 0x0B070903u);
+        case 0: return raw(as_s(__builtin_amdgcn_perm(wh, wl, 0x040C000Cu)) >> (s16x2){8, 8});
+        default: return raw(as_s(__builtin_amdgcn_perm(wh, wl, 0x060C020Cu)) >> (s16x2){8, 8});
+        }
+    }
     __device__ static B load_bnd(const B* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
     __device__ static B zero_bnd() { return 0u; }
 };
@@ -203,9 +217,11 @@ struct Wide32 {
     __device__ static bool eq_lo(V x, V t) { return x.l == t.l; }
     __device__ static bool eq_hi(V x, V t) { return x.h == t.h; }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
-    __device__ static V sval(uint32_t packed)
+    template <int K>
+    __device__ static V prof8(const uint32_t* plo, const uint32_t* phi, int k)
     {
-        return V{(int32_t)(int16_t)(packed & 0xFFFFu), (int32_t)(int16_t)(packed >> 16)};
+        const int sh = (k & 3) * 8;
+        return V{(int32_t)(int8_t)(plo[k / 4] >> sh), (int32_t)(int8_t)(phi[k / 4] >> sh)};
     }
     __device__ static V ltmask(V a, V b) { return V{a.l < b.l ? -1 : 0, a.h < b.h ? -1 : 0}; }
     __device__ static V sel(V m, V a, V b)
@@ -288,6 +304,44 @@ __device__ __forceinline__ void prof_load(const uint8_t* row, int v, uint32_t (&
     }
 }
 
+// Byte profile: the kstride8(K) bytes of virtual lane v, as kstride8(K)/4 dwords
+// (dword d holds columns 4d .. 4d+3).
+template <int K>
+__device__ __forceinline__ void prof_load8(const uint8_t* row, int v, uint32_t (&d)[kstride8(K) / 4])
+{
+    constexpr int Q = K / 16, RS = kremstride8(K);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const uint4 x = *(const uint4*)(row + q * 512 + v * 16);
+        d[4 * q + 0] = x.x; d[4 * q + 1] = x.y; d[4 * q + 2] = x.z; d[4 * q + 3] = x.w;
+    }
+    const uint8_t* r = row + Q * 512 + v * RS;
+    if constexpr (RS == 16) {
+        const uint4 x = *(const uint4*)r;
+        d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y; d[4 * Q + 2] = x.z; d[4 * Q + 3] = x.w;
+    } else if constexpr (RS == 8) {
+        const uint2 x = *(const uint2*)r;
+        d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y;
+    } else if constexpr (RS == 4) {
+        d[4 * Q] = *(const uint32_t*)r;
+    }
+}
+
+// Byte-profile offset j (0..32*kstride8(K)) -> (virtual lane, column within it).
+template <int K>
+__device__ __forceinline__ void prof_slot8(int j, int& v, int& k)
+{
+    constexpr int Q = K / 16, RS = kremstride8(K);
+    if (j < Q * 512) {
+        v = (j & 511) >> 4;
+        k = (j >> 9) * 16 + (j & 15);
+    } else {
+        const int jj = j - Q * 512;
+        v = jj / RS;
+        k = Q * 16 + (jj - v * RS);
+    }
+}
+
 // Profile column index j (0..32*kstride(K)) -> (virtual lane, column within it);
 // k >= K marks padding.
 template <int K>
@@ -337,22 +391,21 @@ struct PassResult {
 //
 // Forward key: (score << 40) | ((2^20-1-row) << 20) | (2^20-1-prio(col)), max wins.
 // Reverse key: ((2^20-1-col) << 20) | (2^20-1-row), max wins (target = score).
-// PF (profile-free): substitution scores come from the (alpha+1)^2 table in the
-// LDS header, addressed by the row code from the ring and per-lane column codes
-// held in registers for the strip; otherwise from the per-strip profile.
-template <class A, int K, bool REV, bool PF>
+// P8: the strip's profile holds int8 scores (prof_row_bytes(K, true) per row).
+template <class A, int K, bool REV, bool P8>
 __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& t, int32_t target,
                                typename A::B* bnd_g)
 {
     using V = typename A::V;
     using Bd = typename A::B;
     constexpr int KS = kstride(K);
+    constexpr int NW = P8 ? kstride8(K) / 4 : KS / 2; // profile dwords of one virtual lane and row
     constexpr int W = kVirtPerPair * K;   // columns of one strip
-    constexpr int WA = kVirtPerPair * KS; // profile columns reserved per row
+    constexpr int WA = kVirtPerPair * KS; // profile columns reserved per row (int16 profile)
     const int lane = threadIdx.x;
     const int g = lane >> 4, l = lane & 15;
     const int alpha = a.alpha;
-    const int rowstride = PF ? (a.alpha + 1) * 2 : WA * 2; // bytes between rows of the score source
+    constexpr int rowstride = prof_row_bytes(K, P8); // bytes between profile rows
     const LaneFix lfix = lane_fix(l);
     uint8_t* gb = sm.group(g);
     int16_t* prof = (int16_t*)gb;
@@ -391,13 +444,10 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         const bool use_bnd = strip > 0;
         const bool write_bnd = live && strip + 1 < strips_g;
 
-        // ---- profile of this strip: prof[a*W + j] = s(code a, column strip_base+j);
-        // profile-free: byte offsets of the lane's column codes in a table row
-        // (low half: virtual lane l, high half: l + 16; the padding code alpha for
-        // columns outside the pair)
-        uint32_t coff[PF ? K : 1];
+        // ---- profile of this strip: row a holds s(code a, column) for the strip's
+        // columns, the padding score for columns outside the pair and in row alpha
         __syncthreads();
-        if constexpr (!PF) {
+        if constexpr (!P8) {
             for (int j = l; j < WA; j += 16) {
                 int vj, kj;
                 prof_slot<K>(j, vj, kj);
@@ -411,13 +461,30 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                     prof[r * WA + j] = A::prof_val((code >= 0 && r < alpha) ? sub[r * alpha + code] : kNeg);
             }
         } else {
+            // four byte columns (one dword) per lane and row
+            uint8_t* prof8 = (uint8_t*)gb;
+            for (int j4 = l; j4 < rowstride / 4; j4 += 16) {
+                int code[4];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int c_lo = strip_base + v_lo * K + k, c_hi = strip_base + v_hi * K + k;
-                int code_lo = alpha, code_hi = alpha;
-                if (live && c_lo < t.nc) code_lo = min((int)lut[t.cols[t.rev ? t.nc_full - 1 - c_lo : c_lo]], alpha);
-                if (live && c_hi < t.nc) code_hi = min((int)lut[t.cols[t.rev ? t.nc_full - 1 - c_hi : c_hi]], alpha);
-                coff[k] = (uint32_t)(code_lo * 2) | ((uint32_t)(code_hi * 2) << 16);
+                for (int b = 0; b < 4; ++b) {
+                    int vj, kj;
+                    prof_slot8<K>(j4 * 4 + b, vj, kj);
+                    const int c = strip_base + vj * K + kj;
+                    code[b] = -1;
+                    if (live && kj < K && c < t.nc) {
+                        code[b] = lut[t.cols[t.rev ? t.nc_full - 1 - c : c]];
+                        if (code[b] >= alpha) code[b] = -1;
+                    }
+                }
+                for (int r = 0; r <= alpha; ++r) {
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int sc = (code[b] >= 0 && r < alpha) ? sub[r * alpha + code[b]] : -128;
+                        w |= (uint32_t)(uint8_t)sc << (8 * b);
+                    }
+                    *(uint32_t*)(prof8 + r * rowstride + j4 * 4) = w;
+                }
             }
         }
         // ---- row ring: rows [-32, 0) padding, [0, 32) now, [32, 64) in flight
@@ -455,25 +522,18 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             // so the compiler will not unroll a runtime trip count); a step past
             // steps_max only meets padding rows
             const int s_end = min(s0 + kChunk, steps_max);
-            // profile words (or the K packed scores) and strip boundary of one step,
-            // requested a step ahead
+            // profile words and strip boundary of one step, requested a step ahead
             struct StepIn {
-                uint32_t plo[PF ? 1 : KS / 2], phi[PF ? 1 : KS / 2];
-                uint32_t sv[PF ? K : 1];
+                uint32_t plo[NW], phi[NW];
                 Bd bnd;
             };
             auto fetch = [&](StepIn& in, const int s, const uint32_t off_lo, const uint32_t off_hi) {
-                if constexpr (!PF) {
+                if constexpr (P8) {
+                    prof_load8<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
+                    prof_load8<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
+                } else {
                     prof_load<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
                     prof_load<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
-                } else {
-                    const uint8_t* tab = (const uint8_t*)sub;
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const uint32_t s_lo = *(const uint16_t*)(tab + off_lo + (coff[k] & 0xFFFFu));
-                        const uint32_t s_hi = *(const uint16_t*)(tab + off_hi + (coff[k] >> 16));
-                        in.sv[k] = s_lo | (s_hi << 16);
-                    }
                 }
                 in.bnd = ring_bnd[s & (kRing - 1)]; // zero unless a strip precedes
             };
@@ -491,7 +551,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     V S;
-                    if constexpr (PF) S = A::sval(in.sv[k]);
+                    if constexpr (P8) S = A::template prof8<K>(plo, phi, k);
                     else S = A::template prof<K>(plo, phi, k);
                     const V D = A::add(Hd, S);
                     Hd = Hcol[k];
@@ -758,23 +818,11 @@ __device__ __forceinline__ void reverse_result(const KernelArgs& a, int64_t p, c
     else { *ref_begin = orow; *query_begin = ocol; }                       // 596-600
 }
 
-// Byte->code table, substitution scores and column maxima into LDS.  With PF the
-// scores form an (alpha+1)^2 table in the policy's representation whose last row
-// and column (the padding code) hold -16384; otherwise the raw alpha^2 scores the
-// profile build reads.
-template <class A, bool PF>
+// Byte->code table, substitution scores and column maxima into LDS.
 __device__ __forceinline__ void stage_tables(const KernelArgs& a, const Smem& sm)
 {
     for (int i = threadIdx.x; i < 256; i += kWave) sm.lut()[i] = a.lut[i];
-    if constexpr (PF) {
-        const int A1 = a.alpha + 1;
-        for (int i = threadIdx.x; i < A1 * A1; i += kWave) {
-            const int r = i / A1, c = i % A1;
-            sm.sub()[i] = A::prof_val((r < a.alpha && c < a.alpha) ? a.sub[r * a.alpha + c] : kNeg);
-        }
-    } else {
-        for (int i = threadIdx.x; i < a.alpha * a.alpha; i += kWave) sm.sub()[i] = a.sub[i];
-    }
+    for (int i = threadIdx.x; i < a.alpha * a.alpha; i += kWave) sm.sub()[i] = a.sub[i];
     for (int i = threadIdx.x; i < a.alpha; i += kWave) sm.colmax()[i] = a.colmax[i];
     __syncthreads();
 }
@@ -787,10 +835,10 @@ __device__ __forceinline__ int64_t slot_pair(const KernelArgs& a, int64_t slot)
 
 // ---------------------------------------------------------------- kernels
 
-template <class A, int K>
+template <class A, int K, bool P8>
 __device__ __forceinline__ Smem make_smem(uint8_t* raw, const KernelArgs& a)
 {
-    return Smem{raw, lds_layout(kVirtPerPair * kstride(K), a.alpha, (int)sizeof(typename A::B))};
+    return Smem{raw, lds_layout(prof_row_bytes(K, P8), a.alpha, (int)sizeof(typename A::B))};
 }
 
 template <class A>
@@ -802,19 +850,19 @@ __device__ __forceinline__ typename A::B* group_bnd(const KernelArgs& a)
 
 // Plain form, forward pass: quads of consecutive pairs in input order, statically
 // strided over the grid (one launch per batch, like driver.cpp:176-185).
-template <int K, bool PF>
+template <int K, bool P8>
 __global__ __launch_bounds__(64) void k_plain_forward(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    const Smem sm = make_smem<Packed16, K>(smem_raw, a);
-    stage_tables<Packed16, PF>(a, sm);
+    const Smem sm = make_smem<Packed16, K, P8>(smem_raw, a);
+    stage_tables(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     Packed16::B* bnd_g = group_bnd<Packed16>(a);
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
     for (int64_t q = blockIdx.x; q < nquads; q += gridDim.x) {
         const int64_t p = slot_pair(a, q * kPairsPerWave + g);
         const Task t = forward_task(a, p);
-        const PassResult r = run_pass<Packed16, K, false, PF>(a, sm, t, 0, bnd_g);
+        const PassResult r = run_pass<Packed16, K, false, P8>(a, sm, t, 0, bnd_g);
         const FwdOut f = forward_result(a, p, r.key);
         if (p >= 0 && l == 0) {
             a.out[kOutScore * a.n + p] = f.score;
@@ -826,12 +874,12 @@ __global__ __launch_bounds__(64) void k_plain_forward(KernelArgs a)
 
 // Plain form, reverse pass: reads the forward results of the same buffer
 // (driver.cpp:199-208, with no host round trip for the block size).
-template <int K, bool PF>
+template <int K, bool P8>
 __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    const Smem sm = make_smem<Packed16, K>(smem_raw, a);
-    stage_tables<Packed16, PF>(a, sm);
+    const Smem sm = make_smem<Packed16, K, P8>(smem_raw, a);
+    stage_tables(a, sm);
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     Packed16::B* bnd_g = group_bnd<Packed16>(a);
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
@@ -844,7 +892,7 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
             f.query_end = a.out[kOutQueryEnd * a.n + p];
         }
         const Task t = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult r = run_pass<Packed16, K, true, PF>(a, sm, t, f.score, bnd_g);
+        const PassResult r = run_pass<Packed16, K, true, P8>(a, sm, t, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, t, r.key, &rb, &qb);
         if (p >= 0 && l == 0 && f.score >= 0) {
@@ -857,12 +905,12 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
 // Batched form: persistent workgroups pull quads of length-sorted pairs from a
 // device work queue and run forward then reverse back to back.  With A = Wide32
 // it recomputes the pairs the int16 kernels queued as possibly saturated.
-template <class A, int K, bool PF>
+template <class A, int K, bool P8>
 __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
 {
     extern __shared__ __align__(16) uint8_t smem_raw[];
-    const Smem sm = make_smem<A, K>(smem_raw, a);
-    stage_tables<A, PF>(a, sm);
+    const Smem sm = make_smem<A, K, P8>(smem_raw, a);
+    stage_tables(a, sm);
     const int l = threadIdx.x & 15;
     const int g = threadIdx.x >> 4;
     typename A::B* bnd_g = group_bnd<A>(a);
@@ -874,10 +922,10 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
         if ((int64_t)q >= nquads) break;
         const int64_t p = slot_pair(a, (int64_t)q * kPairsPerWave + g);
         const Task tf = forward_task(a, p);
-        const PassResult rf = run_pass<A, K, false, PF>(a, sm, tf, 0, bnd_g);
+        const PassResult rf = run_pass<A, K, false, P8>(a, sm, tf, 0, bnd_g);
         const FwdOut f = forward_result(a, p, rf.key);
         const Task tr = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult rr = run_pass<A, K, true, PF>(a, sm, tr, f.score, bnd_g);
+        const PassResult rr = run_pass<A, K, true, P8>(a, sm, tr, f.score, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, tr, rr.key, &rb, &qb);
         if (p >= 0 && l == 0 && f.score >= 0) {
@@ -898,27 +946,28 @@ __global__ void k_zero_u32(uint32_t* p, int n)
 
 // ---------------------------------------------------------------- host side
 
-template <int K, bool PF>
+template <int K, bool P8>
 static const void* kernel_ptr(int kind)
 {
     switch (kind) {
-    case kPlainForward: return (const void*)&k_plain_forward<K, PF>;
-    case kPlainReverse: return (const void*)&k_plain_reverse<K, PF>;
-    case kBatched: return (const void*)&k_batched<Packed16, K, PF>;
+    case kPlainForward: return (const void*)&k_plain_forward<K, P8>;
+    case kPlainReverse: return (const void*)&k_plain_reverse<K, P8>;
+    case kBatched: return (const void*)&k_batched<Packed16, K, P8>;
     default: return nullptr;
     }
 }
 
-template <int K, bool PF>
+template <int K, bool P8>
 static const void* wide_ptr()
 {
-    return (const void*)&k_batched<Wide32, K, PF>;
+    return (const void*)&k_batched<Wide32, K, P8>;
 }
 
-template <int K, bool PF>
+template <int K, bool P8>
 static const void* f16_ptr()
 {
-    return (const void*)&k_batched<PackedF16, K, PF>;
+    if constexpr (P8) return nullptr; // f16 kernels read int16 profiles only
+    else return (const void*)&k_batched<PackedF16, K, false>;
 }
 
 #define GBSW_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
@@ -929,12 +978,12 @@ static const void* f16_ptr()
 // scores below 2048
 #define GBSW_F16_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20)
 
-template <bool PF>
-static const void* kernel_for_pf(int kind, int K)
+template <bool P8>
+static const void* kernel_for_profile(int kind, int K)
 {
     if (kind == kBatchedWide) {
         switch (K) {
-#define GBSW_CASE(k) case k: return wide_ptr<k, PF>();
+#define GBSW_CASE(k) case k: return wide_ptr<k, P8>();
             GBSW_WIDE_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
         default: return nullptr;
@@ -942,24 +991,25 @@ static const void* kernel_for_pf(int kind, int K)
     }
     if (kind == kBatchedF16) {
         switch (K) {
-#define GBSW_CASE(k) case k: return f16_ptr<k, PF>();
+#define GBSW_CASE(k) case k: return f16_ptr<k, P8>();
             GBSW_F16_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
         default: return nullptr;
         }
     }
     switch (K) {
-#define GBSW_CASE(k) case k: return kernel_ptr<k, PF>(kind);
+#define GBSW_CASE(k) case k: return kernel_ptr<k, P8>(kind);
         GBSW_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
     default: return nullptr;
     }
 }
 
-// kind may carry kProfileFree (the instantiation for large alphabets)
+// kind may carry kByteProfile
 static const void* kernel_for(int kind, int K)
 {
-    return (kind & kProfileFree) ? kernel_for_pf<true>(kind & ~kProfileFree, K) : kernel_for_pf<false>(kind, K);
+    return (kind & kByteProfile) ? kernel_for_profile<true>(kind & ~kByteProfile, K)
+                                 : kernel_for_profile<false>(kind, K);
 }
 
 bool kernel_supported(int K) { return kernel_for(kPlainForward, K) != nullptr; }

@@ -7,10 +7,9 @@
 namespace gbsw {
 
 enum KernelKind { kPlainForward = 0, kPlainReverse = 1, kBatched = 2, kBatchedWide = 3, kBatchedF16 = 4 };
-// Flag on a kind: the instantiation that reads scores from an (alpha+1)^2 table
-// instead of a per-strip profile, used for alphabets where profile_free(alpha).
-constexpr int kProfileFree = 0x100;
-inline int with_alphabet(int kind, int alpha) { return profile_free(alpha) ? (kind | kProfileFree) : kind; }
+// Flag on a kind: the instantiation with an int8 profile (no f16 form).
+constexpr int kByteProfile = 0x100;
+inline int with_profile(int kind, bool bytes8) { return bytes8 ? (kind | kByteProfile) : kind; }
 
 // largest score + one substitution an f16 kernel may meet (exact integers in f16)
 constexpr int kF16Exact = 2048;

@@ -278,7 +278,7 @@ int gbsw_server::init_tables(int which)
         t.alpha = build_alphabet(GBSW_PROTEIN, nullptr, nullptr, 0, t.lut);
         for (int c = 0; c < 256; ++c) accept[c] = c < 91;
     }
-    t.kmax = max_k_for(t.alpha);
+    t.kmax = max_k_for(t.alpha, byte_profile_alphabet(t.alpha));
     t.bucket_of_len = bucket_table(t.alpha, slots[0]->ex.kmodel);
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
@@ -558,7 +558,7 @@ void gbsw_server::finish(Slot& s, bool device_ok)
         double ms = 0.0;
         rc = complete_job(s.ex, s.job, &s.sc, &ms);
         bool redone = false;
-        for (const LaunchRec& L : s.ex.launches) redone |= (L.kind & ~kProfileFree) == kBatchedWide;
+        for (const LaunchRec& L : s.ex.launches) redone |= (L.kind & ~kByteProfile) == kBatchedWide;
         if (rc == GBSW_OK && redone) {
             // int32 recompute ran after the copy-back: copy the results again
             hipError_t e = hipMemcpyAsync(s.out_host.ptr, s.out.ptr, sizeof(int32_t) * 5 * (size_t)s.job.n,
@@ -823,7 +823,8 @@ int gbsw_server_submit(gbsw_server* srv, const gbsw_scoring* sc, int64_t n, cons
         probe.max_cols = r->max_cols;
         std::vector<int16_t> sub;
         int max_sub = 0;
-        const int rc = prepare_scoring(probe, sc, sub, &max_sub);
+        bool p8 = false;
+        const int rc = prepare_scoring(probe, sc, sub, &max_sub, &p8);
         if (rc) {
             delete r;
             return rc;
