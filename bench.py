@@ -216,7 +216,8 @@ def main():
     serial = time_mode(batch, scoring, "batched", 1, 1, cells)
     ctx.set_option(gbsw.OPT_STREAMS, 4)
     dom = max(serial["launches"], key=lambda x: x["ms"])
-    kname = "k_batched<%s, %d>" % (KIND_NAMES.get(dom["kind"], "?"), dom["columns_per_lane"])
+    kname = "k_batched<%s, %d, %s>" % (KIND_NAMES.get(dom["kind"] & 0xFF, "?"), dom["columns_per_lane"],
+                                       "true" if dom["kind"] & gbsw.KIND_BYTE_PROFILE else "false")
     achieved = OPS_PER_CELL * dom["cells"] / (dom["ms"] / 1000.0) / 1e12
     agg = OPS_PER_CELL * cells / (batched["device_ms"] / 1000.0) / 1e12
     log("dominant kernel (buckets serialised, %.2f ms/step): %s, %d pairs, %.4g cells, %.3f ms, %.1f GCUPS" %

@@ -247,6 +247,16 @@ int bucket_ids()
     return 2 * nk;
 }
 
+StripWidths strip_widths()
+{
+    StripWidths sw;
+    std::memset(&sw, 0, sizeof(sw));
+    int nk = 0;
+    const int* ks = kernel_k_list(&nk);
+    for (int b = 0; b < std::min(32, 2 * nk); ++b) sw.w[b] = (uint16_t)(kVirtPerPair * ks[b % nk]);
+    return sw;
+}
+
 const std::vector<uint8_t>& bucket_table(int alpha, int model)
 {
     static std::mutex mu;
@@ -422,7 +432,8 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         ex.launches.back().pairs = job.n;
     } else {
         // device binning: keys, then a stable radix sort into the work order
-        GBSW_HIP_TRY(bin_keys(job.ref_off, job.qry_off, job.n, job.bucket_of_len, job.keys, job.vals, st0));
+        GBSW_HIP_TRY(bin_keys(job.ref_off, job.qry_off, job.n, job.bucket_of_len, strip_widths(), job.keys, job.vals,
+                              st0));
         size_t tb = job.sort_temp_bytes;
         GBSW_HIP_TRY(sort_pairs(job.sort_temp, tb, job.keys, job.keys_sorted, job.vals, job.order, (int)job.n,
                                 kSortBits, st0));

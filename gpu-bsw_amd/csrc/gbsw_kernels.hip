@@ -510,6 +510,21 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
         V best = A::zero(), bstep = A::zero(); // bstep: step of the first row reaching best
         int32_t rt_lo = 0x7FFFFFFF, rr_lo = 0, rt_hi = 0x7FFFFFFF, rr_hi = 0; // reverse hits
 
+        // Rows [r0, r0 + 32) of the outgoing boundary, parked in their ring slots,
+        // go to global memory: two coalesced stores per lane instead of one store
+        // per step, issued after the next chunk's loads so that waiting for those
+        // does not wait for the stores.  A slot is flushed (chunk after it was
+        // written) before the row 128 further on enters it (two chunks later).
+        auto flush_bnd = [&](const int r0) {
+            if (write_bnd) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int row = r0 + l + 16 * h;
+                    if (row >= 0 && row < t.nr) bnd_g[row] = ring_bnd[row & (kRing - 1)];
+                }
+            }
+        };
+
         for (int s0 = 0; s0 < steps_max; s0 += kChunk) {
             // refill: rows of chunk (s0/32 + 1) enter the ring, chunk +2 is requested
             {
@@ -518,6 +533,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 pf = row_fetch<A>(t, ck + 1, l, bnd_g, use_bnd);
             }
             __syncthreads();
+            // rows leaving the last virtual lane during the previous chunk
+            if (multi_strip && s0 > 0) flush_bnd(s0 - kChunk - (kVirtPerPair - 1));
             // two steps per iteration (manual unroll: the DPP moves are convergent,
             // so the compiler will not unroll a runtime trip count); a step past
             // steps_max only meets padding rows
@@ -565,12 +582,10 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 Eout = E;
                 Hdiag0 = xH;
 
-                // the last virtual lane hands its column to the next strip
-                if (multi_strip) {
-                    const int row = s - v_hi;
-                    if (write_bnd && l == 15 && row >= 0 && row < t.nr)
-                        bnd_g[row] = A::bnd_out(Hout, Eout);
-                }
+                // the last virtual lane hands its column to the next strip through
+                // its row's ring slot (whose incoming entry was read 31 steps ago);
+                // flush_bnd writes them out a chunk at a time
+                if (multi_strip && write_bnd && l == 15) ring_bnd[(s - v_hi) & (kRing - 1)] = A::bnd_out(Hout, Eout);
 
                 const V rm = tree_max<A, K>(Hcol);
                 if constexpr (!REV) {
@@ -627,6 +642,12 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                 rc_lo += 2;
                 rc_hi += 2;
             }
+        }
+        if (multi_strip) {
+            // the last chunk's rows; complete before the next strip reads them
+            __syncthreads();
+            flush_bnd(((steps_max - 1) & ~(kChunk - 1)) - (kVirtPerPair - 1));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         }
 
         // ---- candidates of this strip

@@ -23,8 +23,12 @@ hipError_t kernel_launch(int kind, int K, int grid, int lds_bytes, hipStream_t s
 hipError_t zero_u32(uint32_t* p, int n, hipStream_t stream);
 
 // gbsw_binning.hip
+// Columns of one strip (32 x K) of each bucket id.
+struct StripWidths {
+    uint16_t w[32];
+};
 hipError_t bin_keys(const uint64_t* ref_off, const uint64_t* qry_off, int64_t n, const uint8_t* bucket_of_len,
-                    uint32_t* keys, int32_t* vals, hipStream_t stream);
+                    const StripWidths& widths, uint32_t* keys, int32_t* vals, hipStream_t stream);
 hipError_t sort_pairs(void* temp, size_t& temp_bytes, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
                       int32_t* vout, int n, int end_bit, hipStream_t stream);
 

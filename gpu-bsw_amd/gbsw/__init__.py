@@ -113,6 +113,10 @@ EXPORTED = (
     "gbsw_context_set_option",
 )
 OPT_STREAMS, OPT_F16, OPT_GEOMETRY = 1, 2, 3
+# launch kinds (gbsw_launch_stat.kind): plain forward, plain reverse, batched
+# int16, batched int32, batched f16; plus KIND_BYTE_PROFILE for int8 profiles
+KIND_PLAIN_FORWARD, KIND_PLAIN_REVERSE, KIND_BATCHED, KIND_BATCHED_WIDE, KIND_BATCHED_F16 = 0, 1, 2, 3, 4
+KIND_BYTE_PROFILE = 256
 
 _lib = None
 

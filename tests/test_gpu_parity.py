@@ -266,3 +266,38 @@ def test_batched_options_give_identical_results(ctx):
             ctx.set_option(gbsw.OPT_F16, 1)
     with pytest.raises(gbsw.GbswError):
         ctx.set_option(gbsw.OPT_STREAMS, 9)
+
+
+def test_geometry_models_give_identical_results(ctx):
+    """Both length -> geometry models (the plain form of protein follows the
+    option too): the same results."""
+    for w in (synth.batched_dna(3000, seed=0x5EED0204), synth.plain_protein(800, seed=0x5EED0205)):
+        exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring)
+        sc = gbsw_scoring(w.alphabet, w.scoring)
+        try:
+            for model in (0, 1):
+                ctx.set_option(gbsw.OPT_GEOMETRY, model)
+                for mode in ("plain", "batched"):
+                    print("case geometry", w.name, "model", model, mode, flush=True)
+                    got = ctx.align_packed(w.ref, w.ref_off, w.query, w.query_off, sc, mode)
+                    assert_same(got, exp, "%s model=%d %s" % (w.name, model, mode))
+        finally:
+            ctx.set_option(gbsw.OPT_GEOMETRY, 1)
+
+
+def test_protein_scores_beyond_int8_use_the_int16_profile(ctx):
+    """BLOSUM62 x 12 (scores -48..132) does not fit the int8 profile: those
+    launches take the int16 profile at a geometry that fits its LDS, BLOSUM62
+    itself the int8 one; both match the oracle."""
+    w = synth.plain_protein(400, seed=0x5EED0206)
+    big = [12 * v for v in BLOSUM62]
+    exp = ol.align_batch(w.ref, w.ref_off, w.query, w.query_off, ol.aa_scoring(big, -30, -4))
+    for mode in ("plain", "batched"):
+        print("case scaled blosum", mode, flush=True)
+        b = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
+        b.align(gbsw.protein_scoring(big, -30, -4), mode)
+        assert_same(b.results(), {f: exp[f] for f in FIELDS}, "blosum62x12/" + mode)
+        assert all(L["kind"] < gbsw.KIND_BYTE_PROFILE for L in b.launches())
+        b.align(gbsw.protein_scoring(BLOSUM62, -6, -1), mode)
+        assert all(L["kind"] >= gbsw.KIND_BYTE_PROFILE for L in b.launches())
+        b.close()
