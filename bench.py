@@ -103,18 +103,22 @@ def time_mode(batch, scoring, mode, warmup, steps, cells):
             "launches": avg, "buckets": st["buckets"]}
 
 
-def serving_bench(ctx, args):
+def serving_bench(args):
     from gbsw import loadgen
     t = time.perf_counter()
     pool = loadgen.Pool(args.serving_requests)
     log("serving pool: %d requests, %d pairs, mean %.3g cells/request, built in %.1f s" %
         (len(pool.requests), sum(r["n"] for r in pool.requests), pool.mean_cells, time.perf_counter() - t))
-    # each request aligned alone: the results every served copy must reproduce
+    # each request aligned alone: the results every served copy must reproduce.
+    # The context is closed before the server starts, so that the server's
+    # streams have the hardware queues to themselves.
     alone = []
+    ctx = gbsw.Context(0)
     for r in pool.requests:
         w = r["workload"]
         res = ctx.align_packed(w.ref, w.ref_off, w.query, w.query_off, pool.scorings[w.alphabet], "batched")
         alone.append(np.stack([res[f] for f in gbsw.FIELDS]))
+    ctx.close()
     mismatches = [0]
 
     def verify(k, out, info):
@@ -229,9 +233,11 @@ def main():
                 "kernel_ms": round(dom["ms"], 3), "ops_per_cell": OPS_PER_CELL,
                 "aggregate_achieved": round(agg, 3), "aggregate_frac": round(agg / PEAK_INT_TOPS_PACKED, 4)}
 
+    batch.close()
+    ctx.close()
     serving = None
     if not args.no_serving:
-        serving = serving_bench(ctx, args)
+        serving = serving_bench(args)
         if not serving["bit_exact_vs_alone"]:
             log("ERROR: %d served requests differ from the same requests aligned alone" % serving["mismatches"])
             sys.exit(1)
@@ -268,8 +274,6 @@ def main():
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    batch.close()
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -127,6 +127,10 @@ int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value)
         if (value != 0 && value != 1) return fail(GBSW_E_ARGUMENT, "f16 option is 0 or 1");
         ctx->ex.f16 = value != 0;
         return GBSW_OK;
+    case GBSW_OPT_SPLIT:
+        if (value != 0 && value != 1) return fail(GBSW_E_ARGUMENT, "split option is 0 or 1");
+        ctx->ex.split = value != 0;
+        return GBSW_OK;
     case GBSW_OPT_GEOMETRY:
         if (value != kModelClassic && value != kModelOccupancy) return fail(GBSW_E_ARGUMENT, "geometry model is 0 or 1");
         ctx->ex.kmodel = (int)value;

@@ -72,6 +72,12 @@ bool default_f16()
     return !(e && std::atoi(e) == 0);
 }
 
+bool default_split()
+{
+    const char* e = std::getenv("GBSW_SPLIT");
+    return !(e && std::atoi(e) == 0);
+}
+
 int default_kmodel()
 {
     const char* e = std::getenv("GBSW_KMODEL");
@@ -148,6 +154,7 @@ int Exec::init(int dev, int ncu)
     num_cu = ncu;
     streams = default_streams();
     f16 = default_f16();
+    split = default_split();
     kmodel = default_kmodel();
     hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
@@ -185,6 +192,7 @@ void Exec::destroy()
         aux[i] = nullptr;
     }
     bnd.release(); queue.release(); status.release(); sub.release(); lut.release(); colmax.release(); redo.release();
+    split_buf.release();
     status_host.release();
     tables_host.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -438,22 +446,48 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         GBSW_HIP_TRY(sort_pairs(job.sort_temp, tb, job.keys, job.keys_sorted, job.vals, job.order, (int)job.n,
                                 kSortBits, st0));
         const size_t nb = job.buckets.size();
-        if (ex.queue.bytes < sizeof(uint32_t) * nb) GBSW_HIP_TRY(ex.queue.reserve(sizeof(uint32_t) * nb));
-        GBSW_HIP_TRY(zero_u32(ex.queue.as<uint32_t>(), (int)nb, st0));
+        // kXcds work counters per bucket (k_batched uses the first)
+        if (ex.queue.bytes < sizeof(uint32_t) * kXcds * nb) GBSW_HIP_TRY(ex.queue.reserve(sizeof(uint32_t) * kXcds * nb));
+        GBSW_HIP_TRY(zero_u32(ex.queue.as<uint32_t>(), (int)(kXcds * nb), st0));
         // Buckets run concurrently on 1 + kAuxStreams streams, the ones whose single
         // work item takes longest (largest K) first, so a bucket with few long
         // pairs overlaps the bulk instead of running alone at the end.
         std::vector<size_t> order(nb);
-        std::vector<int> grids(nb), ldss(nb), kinds(nb), kk(nb);
-        size_t bnd_words = 0;
+        std::vector<int> grids(nb), ldss(nb), kinds(nb), kk(nb), strips(nb, 1);
+        std::vector<size_t> split_at(nb, 0);
+        size_t bnd_words = 0, split_words = 0;
         for (size_t i = 0; i < nb; ++i) {
             order[i] = i;
             const Bucket& bk = job.buckets[i];
             kk[i] = std::min(kclamp, bk.K);
             kinds[i] = with_profile(batched_kind(bk, max_sub, ex.f16 && !p8), p8);
             ldss[i] = lds_for(kk[i], job.alpha, p8);
-            grids[i] = grid_for(ex, kinds[i], kk[i], ldss[i], (bk.count + kPairsPerWave - 1) / kPairsPerWave);
-            bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+            const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
+            strips[i] = std::max(1, (bk.max_cols + kVirtPerPair * kk[i] - 1) / (kVirtPerPair * kk[i]));
+            const int skind = with_profile((kinds[i] & ~kByteProfile) == kBatchedF16 ? kSplitF16 : kSplit, p8);
+            // A split item waits about four chunks for its predecessor's first
+            // rows: worth it only when the bucket has too few quads to keep every
+            // workgroup busy for two rounds (on the 200k-pair DNA mix: 1765 vs 827
+            // GCUPS at 1.4 quads per workgroup; 988 vs 1275 on protein at 2.7).
+            const int64_t full = (int64_t)std::max(1, kernel_occupancy(kinds[i], kk[i], ldss[i])) * ex.num_cu;
+            if (ex.split && strips[i] > 1 && nquads < 2 * full && kernel_has(skind, kk[i])) {
+                // every XCD's counter needs workgroups: a multiple of kXcds, and
+                // two boundary slots per workgroup
+                kinds[i] = skind;
+                int occ = kernel_occupancy(skind, kk[i], ldss[i]);
+                grids[i] = std::max(kXcds, std::max(1, occ) * ex.num_cu / kXcds * kXcds);
+                bnd_words += (size_t)2 * grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+                split_at[i] = split_words;
+                // keys (2 x count u64), done (2 x quads u32), prog (slots u64), rel (slots u32)
+                split_words += 4 * (size_t)bk.count + 2 * (size_t)nquads + 3 * (size_t)(2 * grids[i]);
+            } else {
+                grids[i] = grid_for(ex, kinds[i], kk[i], ldss[i], nquads);
+                bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+            }
+        }
+        if (split_words) {
+            GBSW_HIP_TRY(ex.split_buf.reserve(split_words * sizeof(uint32_t)));
+            GBSW_HIP_TRY(zero_u32(ex.split_buf.as<uint32_t>(), (int)split_words, st0));
         }
         std::sort(order.begin(), order.end(),
                   [&](size_t x, size_t y) { return job.buckets[x].K > job.buckets[y].K; });
@@ -467,14 +501,29 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
             const Bucket& bk = job.buckets[i];
             a.order = job.order + bk.begin;
             a.order_n = bk.count;
-            a.queue = ex.queue.as<uint32_t>() + i;
-            a.sat_limit = (kinds[i] & ~kByteProfile) == kBatchedF16 ? kF16Exact - max_sub : 32767 - max_sub;
+            a.queue = ex.queue.as<uint32_t>() + kXcds * i;
+            const int base_kind = kinds[i] & ~kByteProfile;
+            a.sat_limit = (base_kind == kBatchedF16 || base_kind == kSplitF16) ? kF16Exact - max_sub : 32767 - max_sub;
+            std::memset(&a.split, 0, sizeof(a.split));
+            if (base_kind == kSplit || base_kind == kSplitF16) {
+                uint32_t* w = ex.split_buf.as<uint32_t>() + split_at[i];
+                const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
+                const int64_t nslots = 2 * (int64_t)grids[i];
+                a.split.keys = (uint64_t*)w;
+                a.split.prog = (uint64_t*)(w + 4 * bk.count);
+                a.split.done = w + 4 * bk.count + 2 * nslots;
+                a.split.rel = a.split.done + 2 * nquads;
+                a.split.strips = strips[i];
+                a.split.slots = (int32_t)(nslots / kXcds);
+            }
             hipStream_t st = (j % ns) == 0 ? st0 : ex.aux[j % ns - 1];
             rc = launch_timed(ex, st, kinds[i], kk[i], grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
             if (rc) return rc;
             ex.launches.back().pairs = bk.count;
             ex.launches.back().cells = bk.cells;
-            bnd_at += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+            const int base_kind2 = kinds[i] & ~kByteProfile;
+            bnd_at += (size_t)((base_kind2 == kSplit || base_kind2 == kSplitF16) ? 2 : 1) * grids[i] * kPairsPerWave *
+                      (size_t)a.bnd_rows;
         }
         for (int i = 0; i < ns - 1; ++i) {
             GBSW_HIP_TRY(hipEventRecord(ex.ev_join[i], ex.aux[i]));
@@ -527,6 +576,7 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
         ex.launches[i].ms = t;
     }
     if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
+    if (status[kStatusTimeout]) return fail(GBSW_E_INTERNAL, "a split-form wait timed out");
     return GBSW_OK;
 }
 

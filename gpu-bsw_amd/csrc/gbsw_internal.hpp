@@ -72,7 +72,7 @@ struct Exec {
     hipEvent_t ev_fork = nullptr, ev_join[kAuxStreams] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> kev; // per-launch event pairs
-    DeviceBuffer bnd, queue, status, sub, lut, colmax, redo;
+    DeviceBuffer bnd, queue, status, sub, lut, colmax, redo, split_buf;
     PinnedBuffer status_host;    // status words copied back at the end of a job
     PinnedBuffer tables_host;    // staging of sub / lut / colmax for the async copies
     std::vector<LaunchRec> launches;
@@ -80,6 +80,7 @@ struct Exec {
     bool p8 = false;               // the current job uses the int8 profile
     int streams = kAuxStreams + 1; // streams for the length buckets (1 = serialised)
     bool f16 = true;               // exact f16 kernels where the score bound allows
+    bool split = true;             // multi-strip buckets run their strips on several workgroups
     int kmodel = 1;                // length -> geometry model of new batches (kModel*)
     int init(int device, int num_cu);
     void destroy();

@@ -23,7 +23,20 @@ constexpr int16_t kNeg = -16384;
 enum { kOutScore = 0, kOutRefBegin = 1, kOutRefEnd = 2, kOutQueryBegin = 3, kOutQueryEnd = 4 };
 
 // status[] words written by kernels
-enum { kStatusBadResidue = 0, kStatusSaturated = 1, kStatusRevMiss = 2, kStatusWords = 4 };
+enum { kStatusBadResidue = 0, kStatusSaturated = 1, kStatusRevMiss = 2, kStatusTimeout = 3, kStatusWords = 4 };
+
+// Split form (k_split): the strips of a multi-strip pair run on different
+// workgroups of one XCD, each strip following its predecessor's boundary rows.
+struct SplitArgs {
+    uint64_t* keys;   // 2 x order_n: forward, then reverse winner keys by work slot
+    uint32_t* done;   // 2 x quads: forward, then reverse strip items finished
+    uint64_t* prog;   // per boundary slot: epoch << 32 | boundary rows ready
+    uint32_t* rel;    // per boundary slot: epochs released
+    int32_t strips;   // strip items per quad and pass
+    int32_t slots;    // boundary slots per XCD
+};
+constexpr int kXcds = 8;                 // workgroup w runs on XCD w % 8
+constexpr uint64_t kSpinTicks = 200000000ull; // 2 s of the 100 MHz s_memrealtime clock
 
 struct KernelArgs {
     const uint8_t* ref;
@@ -47,6 +60,7 @@ struct KernelArgs {
     int32_t gap_open;          // <= 0
     int32_t gap_extend;        // <= 0
     int32_t sat_limit;         // scores >= this may have saturated int16
+    SplitArgs split;           // k_split only
 };
 
 // Dynamic LDS layout of one 64-lane workgroup; W = profile columns per row

@@ -391,10 +391,68 @@ struct PassResult {
 //
 // Forward key: (score << 40) | ((2^20-1-row) << 20) | (2^20-1-prio(col)), max wins.
 // Reverse key: ((2^20-1-col) << 20) | (2^20-1-row), max wins (target = score).
+// One strip of the split form: its boundary input (the previous strip's rows,
+// with that slot's progress word) and output.
+struct StripIO {
+    int strip;
+    const void* bnd_in;
+    void* bnd_out;
+    const uint64_t* prog_in;
+    uint64_t* prog_out;
+    uint64_t epoch_in, epoch_out; // epoch << 32
+};
+
+// Wave-uniform wait until *w >= target.  Bounded: after kSpinTicks the timeout
+// status word is set and the wave goes on (the host then reports the batch as
+// failed), and once that word is set every wait returns at once, so a stalled
+// pipeline drains instead of timing out wait by wait.
+template <class T>
+__device__ __forceinline__ void wait_at_least(const T* w, T target, int32_t* status)
+{
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+        uint64_t v = 0;
+        int32_t stop = 0;
+        if (threadIdx.x == 0) {
+            v = (uint64_t)__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            stop = __hip_atomic_load(status + kStatusTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+        if ((((uint64_t)hi << 32) | lo) >= (uint64_t)target) break;
+        if (__builtin_amdgcn_readfirstlane(stop) != 0) break;
+        if (__builtin_amdgcn_s_memrealtime() - t0 > kSpinTicks) {
+            if (threadIdx.x == 0) atomicOr(status + kStatusTimeout, 1);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ void wait_word(const uint64_t* w, uint64_t target, int32_t* status)
+{
+    wait_at_least<uint64_t>(w, target, status);
+}
+__device__ __forceinline__ void wait_u32(const uint32_t* w, uint32_t target, int32_t* status)
+{
+    wait_at_least<uint32_t>(w, target, status);
+}
+
+// The wave's earlier global stores and atomics have completed (reached L2),
+// then *w = v.  Readers on the same XCD see the stores once they see v.
+__device__ __forceinline__ void publish_word(uint64_t* w, uint64_t v)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (threadIdx.x == 0) __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // P8: the strip's profile holds int8 scores (prof_row_bytes(K, true) per row).
-template <class A, int K, bool REV, bool P8>
+// SPLIT: run strip io.strip only, reading the previous strip's boundary rows from
+// io.bnd_in as its producer publishes them and handing this strip's rows on
+// through io.bnd_out; otherwise every strip, with the boundary in bnd_g.
+template <class A, int K, bool REV, bool P8, bool SPLIT = false>
 __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& t, int32_t target,
-                               typename A::B* bnd_g)
+                               typename A::B* bnd_g, const StripIO& io = StripIO{})
 {
     using V = typename A::V;
     using Bd = typename A::B;
@@ -438,11 +496,22 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     const int v_lo = l, v_hi = l + 16;
     uint64_t cand_lo = 0, cand_hi = 0;
 
-    for (int strip = 0; strip < strips_max; ++strip) {
+    Bd* bnd_rd = SPLIT ? (Bd*)io.bnd_in : bnd_g;
+    Bd* bnd_wr = SPLIT ? (Bd*)io.bnd_out : bnd_g;
+    const int strip_lo = SPLIT ? io.strip : 0;
+    const int strip_hi = SPLIT ? min(io.strip + 1, strips_max) : strips_max;
+    for (int strip = strip_lo; strip < strip_hi; ++strip) {
         const int strip_base = strip * W;
         const bool live = strip < strips_g;
         const bool use_bnd = strip > 0;
         const bool write_bnd = live && strip + 1 < strips_g;
+        // split form: boundary rows [0, n) of the previous strip are readable once
+        // its progress word reaches epoch_in | n
+        auto need_rows = [&](const int n) {
+            if constexpr (SPLIT) {
+                if (strip > 0) wait_word(io.prog_in, io.epoch_in | (uint64_t)(uint32_t)n, a.status);
+            }
+        };
 
         // ---- profile of this strip: row a holds s(code a, column) for the strip's
         // columns, the padding score for columns outside the pair and in row alpha
@@ -493,10 +562,11 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             pad.b0 = pad.b1 = -1;
             pad.d0 = pad.d1 = A::zero_bnd();
             row_store<A>(pad, -1, l, ring_code, ring_bnd, lut, alpha, rowstride);
-            RowFetch<A> f0 = row_fetch<A>(t, 0, l, bnd_g, use_bnd);
+            need_rows(2 * kChunk);
+            RowFetch<A> f0 = row_fetch<A>(t, 0, l, bnd_rd, use_bnd);
             row_store<A>(f0, 0, l, ring_code, ring_bnd, lut, alpha, rowstride);
         }
-        RowFetch<A> pf = row_fetch<A>(t, 1, l, bnd_g, use_bnd);
+        RowFetch<A> pf = row_fetch<A>(t, 1, l, bnd_rd, use_bnd);
         __syncthreads();
 
 
@@ -520,7 +590,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int row = r0 + l + 16 * h;
-                    if (row >= 0 && row < t.nr) bnd_g[row] = ring_bnd[row & (kRing - 1)];
+                    if (row >= 0 && row < t.nr) bnd_wr[row] = ring_bnd[row & (kRing - 1)];
                 }
             }
         };
@@ -530,7 +600,13 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             {
                 const int ck = s0 / kChunk + 1;
                 row_store<A>(pf, ck, l, ring_code, ring_bnd, lut, alpha, rowstride);
-                pf = row_fetch<A>(t, ck + 1, l, bnd_g, use_bnd);
+                if constexpr (SPLIT) {
+                    // the rows flushed one chunk ago have long been stored: publish
+                    if (s0 > kChunk && strip + 1 < strips_max)
+                        publish_word(io.prog_out, io.epoch_out | (uint64_t)(uint32_t)(s0 - 2 * kChunk + 1));
+                }
+                need_rows((ck + 2) * kChunk);
+                pf = row_fetch<A>(t, ck + 1, l, bnd_rd, use_bnd);
             }
             __syncthreads();
             // rows leaving the last virtual lane during the previous chunk
@@ -647,7 +723,11 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             // the last chunk's rows; complete before the next strip reads them
             __syncthreads();
             flush_bnd(((steps_max - 1) & ~(kChunk - 1)) - (kVirtPerPair - 1));
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if constexpr (SPLIT) {
+                if (strip + 1 < strips_max) publish_word(io.prog_out, io.epoch_out | 0xFFFFFFFFull);
+            } else {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            }
         }
 
         // ---- candidates of this strip
@@ -741,8 +821,9 @@ __device__ __forceinline__ Task forward_task(const KernelArgs& a, int64_t p)
 }
 
 // Decodes the forward key.  A score at or above sat_limit may have saturated the
-// int16 cells: the pair is queued for the int32 kernel and reported as score -1.
-__device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p, uint64_t key)
+// int16 cells: the pair is queued for the int32 kernel (when record_redo) and
+// reported as score -1.
+__device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p, uint64_t key, bool record_redo = true)
 {
     FwdOut o{0, 0, 0};
     if (p < 0 || key == 0) return o;
@@ -752,7 +833,7 @@ __device__ __forceinline__ FwdOut forward_result(const KernelArgs& a, int64_t p,
     const int32_t row = (int32_t)(0xFFFFFu - (uint32_t)((key >> 20) & 0xFFFFFu));
     const int32_t col = (int32_t)inv_prio(0xFFFFFu - (uint32_t)(key & 0xFFFFFu));
     if (score >= a.sat_limit) {
-        if ((threadIdx.x & 15) == 0) {
+        if (record_redo && (threadIdx.x & 15) == 0) {
             const int32_t slot = atomicAdd(a.status + kStatusSaturated, 1);
             a.redo[slot] = (int32_t)p;
         }
@@ -959,6 +1040,123 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
     }
 }
 
+// Split form of k_batched for buckets of multi-strip pairs: a quad's strips run
+// on different workgroups, pipelined, instead of one after another on one.
+//
+// Workgroup w serves XCD x = w % 8 (where the hardware places it), with its own
+// work counter a.queue[x] and boundary slots, so producer and consumer of a
+// boundary share an L2.  XCD x owns quads x, x + 8, ...; its items are first
+// (quad, strip) of the forward pass for all its quads, then the same for the
+// reverse pass (a.split.strips items per quad and pass; strips past a quad's
+// count are empty items).  Item i writes its outgoing boundary into slot
+// i % slots (epoch i / slots) and reads the previous strip's from slot (i - 1),
+// following that slot's progress word.  A slot is released once per epoch, in
+// epoch order, by the strip that read it (or by its producer when no strip
+// follows); a producer waits for the release of the epoch before.  Every wait is
+// for an item taken from the counter earlier, which holds its workgroup until it
+// finishes, so the waits end; each is bounded by kSpinTicks besides.
+//
+// Winner keys reduce with 64-bit atomicMax per pair.  A quad's reverse items
+// start once its forward items are all done; the item completing the reverse
+// count writes the quad's results.
+template <class A, int K, bool P8>
+__global__ __launch_bounds__(64) void k_split(KernelArgs a)
+{
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    const Smem sm = make_smem<A, K, P8>(smem_raw, a);
+    stage_tables(a, sm);
+    const int l = threadIdx.x & 15;
+    const int g = threadIdx.x >> 4;
+    const int x = (int)(blockIdx.x % kXcds);
+    const SplitArgs& sp = a.split;
+    const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
+    const int64_t nq_x = nquads > x ? (nquads - x + kXcds - 1) / kXcds : 0;
+    const int64_t per_pass = nq_x * sp.strips;
+    typename A::B* bnd = (typename A::B*)a.bnd;
+    const int64_t slot0 = (int64_t)x * sp.slots;
+    auto slot_bnd = [&](int64_t i) {
+        return bnd + ((slot0 + i % sp.slots) * kPairsPerWave + g) * a.bnd_rows;
+    };
+    // release epoch i / slots of slot i % slots, after the epoch before it
+    auto release = [&](int64_t i) {
+        uint32_t* r = sp.rel + slot0 + i % sp.slots;
+        const uint32_t e = (uint32_t)(i / sp.slots);
+        wait_u32(r, e, a.status);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (threadIdx.x == 0) __hip_atomic_store(r, e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    for (;;) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.queue + x, 1u);
+        const int64_t i = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)it);
+        if (i >= 2 * per_pass) break;
+        const int rev = i >= per_pass;
+        const int64_t k = rev ? i - per_pass : i;
+        const int64_t q = (int64_t)x + (int64_t)kXcds * (k / sp.strips);
+        const int s = (int)(k % sp.strips);
+        const int64_t ws = q * kPairsPerWave + g; // work slot of this group's pair
+        const int64_t p = slot_pair(a, ws);
+        uint64_t* fkey = sp.keys;
+        uint64_t* rkey = sp.keys + a.order_n;
+
+        Task t;
+        FwdOut f{0, 0, 0};
+        if (!rev) {
+            t = forward_task(a, p);
+        } else {
+            // the quad's forward strips have all finished
+            wait_u32(sp.done + q, (uint32_t)sp.strips, a.status);
+            const uint64_t fk = p >= 0 ? __hip_atomic_load(fkey + ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            f = forward_result(a, p, fk, s == 0);
+            t = reverse_task(a, p, f, column_bound(a, sm, p, f));
+            // strips past the first that reached the target for every pair of the
+            // quad hold larger columns only: nothing left to find
+            if (p >= 0 && t.nr > 0 && t.nc > 0 &&
+                __hip_atomic_load(rkey + ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                t.nr = 0;
+        }
+        // strips of the quad (wave maximum)
+        int st = (t.nr > 0 && t.nc > 0) ? (t.nc + kVirtPerPair * K - 1) / (kVirtPerPair * K) : 0;
+#pragma unroll
+        for (int o = 16; o < 64; o <<= 1) st = max(st, __shfl_xor(st, o));
+        st = __builtin_amdgcn_readfirstlane(st);
+        if (s < st) {
+            wait_u32(sp.rel + slot0 + i % sp.slots, (uint32_t)(i / sp.slots), a.status); // slot free
+            StripIO io;
+            io.strip = s;
+            io.bnd_in = s > 0 ? (const void*)slot_bnd(i - 1) : nullptr;
+            io.bnd_out = (void*)slot_bnd(i);
+            io.prog_in = s > 0 ? sp.prog + slot0 + (i - 1) % sp.slots : nullptr;
+            io.prog_out = sp.prog + slot0 + i % sp.slots;
+            io.epoch_in = s > 0 ? (uint64_t)((i - 1) / sp.slots) << 32 : 0;
+            io.epoch_out = (uint64_t)(i / sp.slots) << 32;
+            PassResult r;
+            if (!rev) r = run_pass<A, K, false, P8, true>(a, sm, t, 0, nullptr, io);
+            else r = run_pass<A, K, true, P8, true>(a, sm, t, f.score, nullptr, io);
+            if (p >= 0 && l == 0 && r.key != 0) atomicMax((unsigned long long*)((rev ? rkey : fkey) + ws), r.key);
+        }
+        if (s > 0) release(i - 1);             // done with the previous strip's rows
+        if (s == sp.strips - 1) release(i);    // no strip follows
+        // count the item; the last reverse item of the quad writes its results
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        uint32_t before = 0;
+        if (threadIdx.x == 0) before = atomicAdd(sp.done + (rev ? nquads : 0) + q, 1u);
+        before = (uint32_t)__builtin_amdgcn_readfirstlane((int)before);
+        if (rev && before == (uint32_t)sp.strips - 1) {
+            const uint64_t rk = p >= 0 ? __hip_atomic_load(rkey + ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            int32_t rb, qb;
+            reverse_result(a, p, f, t, rk, &rb, &qb);
+            if (p >= 0 && l == 0 && f.score >= 0) {
+                a.out[kOutScore * a.n + p] = f.score;
+                a.out[kOutRefEnd * a.n + p] = f.ref_end;
+                a.out[kOutQueryEnd * a.n + p] = f.query_end;
+                a.out[kOutRefBegin * a.n + p] = rb;
+                a.out[kOutQueryBegin * a.n + p] = qb;
+            }
+        }
+    }
+}
+
 __global__ void k_zero_u32(uint32_t* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -985,6 +1183,14 @@ static const void* wide_ptr()
 }
 
 template <int K, bool P8>
+static const void* split_ptr(bool f16)
+{
+    if (!f16) return (const void*)&k_split<Packed16, K, P8>;
+    if constexpr (P8) return nullptr;
+    else return (const void*)&k_split<PackedF16, K, false>;
+}
+
+template <int K, bool P8>
 static const void* f16_ptr()
 {
     if constexpr (P8) return nullptr; // f16 kernels read int16 profiles only
@@ -998,6 +1204,9 @@ static const void* f16_ptr()
 // f16 kernels: the geometries whose single strip (32 x K columns) keeps DNA-like
 // scores below 2048
 #define GBSW_F16_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20)
+
+// split kernels: the geometries multi-strip buckets use
+#define GBSW_SPLIT_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16)
 
 template <bool P8>
 static const void* kernel_for_profile(int kind, int K)
@@ -1014,6 +1223,14 @@ static const void* kernel_for_profile(int kind, int K)
         switch (K) {
 #define GBSW_CASE(k) case k: return f16_ptr<k, P8>();
             GBSW_F16_K_LIST(GBSW_CASE)
+#undef GBSW_CASE
+        default: return nullptr;
+        }
+    }
+    if (kind == kSplit || kind == kSplitF16) {
+        switch (K) {
+#define GBSW_CASE(k) case k: return split_ptr<k, P8>(kind == kSplitF16);
+            GBSW_SPLIT_K_LIST(GBSW_CASE)
 #undef GBSW_CASE
         default: return nullptr;
         }

@@ -6,7 +6,15 @@
 
 namespace gbsw {
 
-enum KernelKind { kPlainForward = 0, kPlainReverse = 1, kBatched = 2, kBatchedWide = 3, kBatchedF16 = 4 };
+enum KernelKind {
+    kPlainForward = 0,
+    kPlainReverse = 1,
+    kBatched = 2,
+    kBatchedWide = 3,
+    kBatchedF16 = 4,
+    kSplit = 5,    // k_split, int16 cells
+    kSplitF16 = 6, // k_split, f16 cells
+};
 // Flag on a kind: the instantiation with an int8 profile (no f16 form).
 constexpr int kByteProfile = 0x100;
 inline int with_profile(int kind, bool bytes8) { return bytes8 ? (kind | kByteProfile) : kind; }

@@ -90,6 +90,13 @@ int gbsw_context_destroy(gbsw_context* ctx);
  *          gbsw_batch_align(GBSW_BATCHED), and to the plain form for protein
  *          (the plain form uses 0 for DNA).  Results are identical. */
 #define GBSW_OPT_GEOMETRY 3
+/* SPLIT:   1 (default; GBSW_SPLIT=0 turns it off): length buckets of pairs that
+ *          span several strips, with too few pairs to occupy the GPU twice
+ *          over, run each strip as its own work item, on several workgroups of
+ *          one XCD in a pipeline (a strip follows the boundary rows of the one
+ *          before as they are written); 0: one workgroup runs a pair's strips
+ *          one after another.  Results are identical. */
+#define GBSW_OPT_SPLIT 4
 int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value);
 
 /* Message of the last error on this thread ("" if none). */

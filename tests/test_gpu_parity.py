@@ -147,6 +147,31 @@ def test_multi_strip_protein_mutated(ctx):
     run_both(ctx, rb, ro, qb, qo, w.alphabet, w.scoring, "long_protein_24_cut")
 
 
+def test_split_form_runs_multi_strip_buckets(ctx):
+    """Multi-strip DNA and protein buckets run as split items (several
+    workgroups per pair, pipelined on the boundary rows), and match the oracle;
+    the small-K geometry model makes many strips per pair."""
+    refs, qrys = rand_pairs(21, 300, 600, 2000, ["ACGT", "AC"], 300, 1200)
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    exp = oracle_results(rb, ro, qb, qo, gbsw.DNA, (3, -3, -6, -1))
+    w = synth.long_protein(24, seed=0x5EED0207)
+    prefs = [bytes(w.pair(k)[0][:2500]) for k in range(w.n)]
+    pqrys = [bytes(w.pair(k)[1][:1200]) for k in range(w.n)]
+    prb, pro = gbsw.pack(prefs)
+    pqb, pqo = gbsw.pack(pqrys)
+    pexp = oracle_results(prb, pro, pqb, pqo, gbsw.PROTEIN, w.scoring)
+    for label, args, alphabet, scoring, want in (("dna", (rb, ro, qb, qo), gbsw.DNA, (3, -3, -6, -1), exp),
+                                                 ("protein", (prb, pro, pqb, pqo), gbsw.PROTEIN, w.scoring, pexp)):
+        print("case split", label, flush=True)
+        b = ctx.batch(alphabet, *args)
+        b.align(gbsw_scoring(alphabet, scoring), "batched")
+        assert_same(b.results(), want, "split/" + label)
+        kinds = {L["kind"] & 0xFF for L in b.launches()}
+        assert kinds & {gbsw.KIND_SPLIT, gbsw.KIND_SPLIT_F16}, kinds
+        b.close()
+
+
 def test_config0_plain_dna_full(ctx):
     w = synth.plain_dna()
     run_both(ctx, w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring, "config0_full")
@@ -249,21 +274,24 @@ def test_dropin_cpp_caller(tmp_path):
 
 
 def test_batched_options_give_identical_results(ctx):
-    """Serialised or concurrent buckets, f16 kernels on or off: the same results
-    (the f16 kernels run only where their exact range holds)."""
+    """Serialised or concurrent buckets, f16 kernels on or off, multi-strip
+    pairs split over workgroups or not: the same results (the f16 kernels run
+    only where their exact range holds)."""
     for w in (synth.batched_dna(3000, seed=0x5EED0202), synth.plain_protein(600, seed=0x5EED0203)):
         exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring)
         sc = gbsw_scoring(w.alphabet, w.scoring)
         try:
-            for streams, f16 in ((4, 1), (1, 1), (4, 0), (1, 0)):
-                print("case options", w.name, "streams", streams, "f16", f16, flush=True)
+            for streams, f16, split in ((4, 1, 1), (1, 1, 1), (4, 0, 1), (1, 0, 0), (4, 1, 0)):
+                print("case options", w.name, "streams", streams, "f16", f16, "split", split, flush=True)
                 ctx.set_option(gbsw.OPT_STREAMS, streams)
                 ctx.set_option(gbsw.OPT_F16, f16)
+                ctx.set_option(gbsw.OPT_SPLIT, split)
                 got = ctx.align_packed(w.ref, w.ref_off, w.query, w.query_off, sc, "batched")
-                assert_same(got, exp, "%s streams=%d f16=%d" % (w.name, streams, f16))
+                assert_same(got, exp, "%s streams=%d f16=%d split=%d" % (w.name, streams, f16, split))
         finally:
             ctx.set_option(gbsw.OPT_STREAMS, 4)
             ctx.set_option(gbsw.OPT_F16, 1)
+            ctx.set_option(gbsw.OPT_SPLIT, 1)
     with pytest.raises(gbsw.GbswError):
         ctx.set_option(gbsw.OPT_STREAMS, 9)
 
