@@ -47,7 +47,9 @@ MI355X_CU, MI355X_GHZ, OPS_PER_CELL = 256, 2.4, 12
 PEAK_INT_TOPS_PACKED = MI355X_CU * 64 * MI355X_GHZ * 2 / 1000.0  # 78.6 TOP/s (int16x2)
 
 
-KIND_NAMES = {2: "Packed16", 3: "Wide32", 4: "PackedF16"}
+# launch kind -> (kernel, cell policy)
+KIND_NAMES = {2: ("k_batched", "Packed16"), 3: ("k_batched", "Wide32"), 4: ("k_batched", "PackedF16"),
+              5: ("k_split", "Packed16"), 6: ("k_split", "PackedF16")}
 
 
 def log(*a):
@@ -101,6 +103,43 @@ def time_mode(batch, scoring, mode, warmup, steps, cells):
     return {"wall_s": wall, "ms_per_step": 1000.0 * wall / steps, "gcups": cells * steps / wall / 1e9,
             "device_ms": float(np.mean(device_ms)), "device_gcups": cells / (np.mean(device_ms) / 1000.0) / 1e9,
             "launches": avg, "buckets": st["buckets"]}
+
+
+def config_bench(args):
+    """The other workloads: 10k plain DNA and 20k protein (plain and batched
+    forms, bit-exact between them) and the 50k long-protein set (batched)."""
+    from gbsw.scoring import BLOSUM62
+    out = {}
+    ctx = gbsw.Context(0)
+    for name, make, plain in (("plain_dna_10k", lambda: synth.plain_dna(), True),
+                              ("plain_protein_20k", lambda: synth.plain_protein(), True),
+                              ("long_protein", lambda: synth.long_protein(args.long_pairs), False)):
+        t = time.perf_counter()
+        w = make()
+        cells = w.cells()
+        gen_s = time.perf_counter() - t
+        sc = gbsw.dna_scoring(*w.scoring) if w.alphabet == gbsw.DNA else gbsw.protein_scoring(BLOSUM62, *w.scoring)
+        b = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
+        rb = time_mode(b, sc, "batched", 1, 3 if plain else 2, cells)
+        res_b = b.results()
+        d = {"pairs": w.n, "cells": cells, "batched_gcups": round(rb["gcups"], 2),
+             "batched_device_gcups": round(rb["device_gcups"], 2), "batched_ms": round(rb["ms_per_step"], 3),
+             "int32_redone_pairs": int(sum(L["pairs"] for L in rb["launches"]
+                                           if L["kind"] & 0xFF == gbsw.KIND_BATCHED_WIDE))}
+        if plain:
+            rp = time_mode(b, sc, "plain", 1, 2, cells)
+            res_p = b.results()
+            d.update({"plain_gcups": round(rp["gcups"], 2), "plain_ms": round(rp["ms_per_step"], 3),
+                      "batched_over_plain": round(rb["gcups"] / rp["gcups"], 3),
+                      "bit_exact_plain_vs_batched": all(np.array_equal(res_b[f], res_p[f]) for f in gbsw.FIELDS)})
+        log("%s: %d pairs, %.4g cells (generated in %.1f s): batched %.1f GCUPS%s" %
+            (name, w.n, cells, gen_s, rb["gcups"],
+             ", plain %.1f GCUPS, bit-exact %s" % (d["plain_gcups"], d["bit_exact_plain_vs_batched"]) if plain else ""))
+        out[name] = d
+        b.close()
+        del w
+    ctx.close()
+    return out
 
 
 def serving_bench(args):
@@ -162,6 +201,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 10k DNA / 20k protein / long-protein lines")
+    ap.add_argument("--long-pairs", type=int, default=50_000, help="pairs of the long-protein set")
     ap.add_argument("--serving-window", type=float, default=5.0, help="seconds per offered load")
     ap.add_argument("--serving-loads", default="0.25,0.5,0.75,1.0")
     ap.add_argument("--serving-requests", type=int, default=512, help="request pool size")
@@ -220,8 +261,9 @@ def main():
     serial = time_mode(batch, scoring, "batched", 1, 1, cells)
     ctx.set_option(gbsw.OPT_STREAMS, 4)
     dom = max(serial["launches"], key=lambda x: x["ms"])
-    kname = "k_batched<%s, %d, %s>" % (KIND_NAMES.get(dom["kind"] & 0xFF, "?"), dom["columns_per_lane"],
-                                       "true" if dom["kind"] & gbsw.KIND_BYTE_PROFILE else "false")
+    kfn, kpol = KIND_NAMES.get(dom["kind"] & 0xFF, ("?", "?"))
+    kname = "%s<%s, %d, %s>" % (kfn, kpol, dom["columns_per_lane"],
+                                "true" if dom["kind"] & gbsw.KIND_BYTE_PROFILE else "false")
     achieved = OPS_PER_CELL * dom["cells"] / (dom["ms"] / 1000.0) / 1e12
     agg = OPS_PER_CELL * cells / (batched["device_ms"] / 1000.0) / 1e12
     log("dominant kernel (buckets serialised, %.2f ms/step): %s, %d pairs, %.4g cells, %.3f ms, %.1f GCUPS" %
@@ -235,6 +277,13 @@ def main():
 
     batch.close()
     ctx.close()
+    configs = None
+    if not args.no_configs:
+        configs = config_bench(args)
+        bad = [k for k, v in configs.items() if v.get("bit_exact_plain_vs_batched") is False]
+        if bad:
+            log("ERROR: plain and batched results differ on %s" % ", ".join(bad))
+            sys.exit(1)
     serving = None
     if not args.no_serving:
         serving = serving_bench(args)
@@ -271,6 +320,7 @@ def main():
                                               "ms_per_step": round(plain["ms_per_step"], 3)},
         "batched_over_plain": None if plain is None else round(batched["gcups"] / plain["gcups"], 3),
         "bit_exact_plain_vs_batched": plain is not None,
+        "configs": configs,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
