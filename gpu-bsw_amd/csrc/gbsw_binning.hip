@@ -1,6 +1,6 @@
 // gbsw_binning.hip -- device-side length binning of a batch (batched form).
 //
-// Every pair gets a 21-bit key: bucket id of its shorter length (5 bits), then
+// Every pair gets a 22-bit key: bucket id of its shorter length (6 bits), then
 // 31 - (strips - 1) (5 bits; strips of the bucket's width its shorter length
 // spans, capped at 32), then 2047 - (longer length / 32) (11 bits).  A stable
 // radix sort of (key, pair index) yields the work order: buckets in ascending
@@ -24,7 +24,7 @@ __global__ void k_bin_keys(const uint64_t* ref_off, const uint64_t* qry_off, int
     const uint32_t la = (uint32_t)(ref_off[p + 1] - ref_off[p]);
     const uint32_t lb = (uint32_t)(qry_off[p + 1] - qry_off[p]);
     const uint32_t m = la < lb ? la : lb, M = la < lb ? lb : la;
-    const uint32_t b = bucket_of_len[m < 65535u ? m : 65535u] & 31u;
+    const uint32_t b = bucket_of_len[m < 65535u ? m : 65535u] & 63u;
     const uint32_t w = widths.w[b] ? widths.w[b] : 1u;
     const uint32_t strips = m ? (m + w - 1) / w : 1u;
     const uint32_t sk = 31u - (strips < 32u ? strips - 1u : 31u);

@@ -252,7 +252,7 @@ int bucket_ids()
 {
     int nk = 0;
     kernel_k_list(&nk);
-    return 2 * nk;
+    return 3 * nk;
 }
 
 StripWidths strip_widths()
@@ -261,7 +261,7 @@ StripWidths strip_widths()
     std::memset(&sw, 0, sizeof(sw));
     int nk = 0;
     const int* ks = kernel_k_list(&nk);
-    for (int b = 0; b < std::min(32, 2 * nk); ++b) sw.w[b] = (uint16_t)(kVirtPerPair * ks[b % nk]);
+    for (int b = 0; b < std::min(64, bucket_ids()); ++b) sw.w[b] = (uint16_t)(kVirtPerPair * ks[b % nk]);
     return sw;
 }
 
@@ -284,10 +284,13 @@ const std::vector<uint8_t>& bucket_table(int alpha, int model)
         const int K = kblk[(size_t)((m + 31) / 32)];
         int idx = 0;
         while (idx < nk && ks[idx] != K) ++idx;
-        // pairs spanning several strips get buckets of their own: a single-strip
-        // bucket then bounds its scores by 32 x K columns
+        // pairs spanning several strips get buckets of their own, so a single-strip
+        // bucket bounds its scores by 32 x K columns; those up to kMultiShortCols
+        // columns apart from longer ones, whose bound (with DNA scores up to 3)
+        // still allows the f16 kernels
         const bool multi = m > kVirtPerPair * K;
-        t[(size_t)m] = (uint8_t)(idx + (multi ? nk : 0));
+        const int cls = !multi ? 0 : m <= kMultiShortCols ? 1 : 2;
+        t[(size_t)m] = (uint8_t)(idx + cls * nk);
     }
     return cache.emplace(key, std::move(t)).first->second;
 }
@@ -326,11 +329,12 @@ void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry)
 int batched_grid_cap(const Exec& ex, int K, int alpha)
 {
     // the f16 and int16 kernels of a geometry share its LDS layout; take the
-    // smaller occupancy so the bound holds for either
+    // larger occupancy, so that a launch's grid (hence its boundary rows) never
+    // exceeds what admission reserved for it
     const bool p8 = byte_profile_alphabet(alpha);
     const int lds = lds_for(K, alpha, p8);
     int occ = kernel_occupancy(with_profile(kBatched, p8), K, lds);
-    if (!p8 && kernel_has(kBatchedF16, K)) occ = std::min(occ, kernel_occupancy(kBatchedF16, K, lds));
+    if (!p8 && kernel_has(kBatchedF16, K)) occ = std::max(occ, kernel_occupancy(kBatchedF16, K, lds));
     return std::max(1, occ) * ex.num_cu;
 }
 
@@ -471,18 +475,51 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
             // GCUPS at 1.4 quads per workgroup; 988 vs 1275 on protein at 2.7).
             const int64_t full = (int64_t)std::max(1, kernel_occupancy(kinds[i], kk[i], ldss[i])) * ex.num_cu;
             if (ex.split && strips[i] > 1 && nquads < 2 * full && kernel_has(skind, kk[i])) {
-                // every XCD's counter needs workgroups: a multiple of kXcds, and
-                // two boundary slots per workgroup
+                // every XCD's counter needs workgroups: a multiple of kXcds, no
+                // more than the items a pipelined quad can keep busy, and two
+                // boundary slots per workgroup
                 kinds[i] = skind;
-                int occ = kernel_occupancy(skind, kk[i], ldss[i]);
-                grids[i] = std::max(kXcds, std::max(1, occ) * ex.num_cu / kXcds * kXcds);
-                bnd_words += (size_t)2 * grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
-                split_at[i] = split_words;
-                // keys (2 x count u64), done (2 x quads u32), prog (slots u64), rel (slots u32)
-                split_words += 4 * (size_t)bk.count + 2 * (size_t)nquads + 3 * (size_t)(2 * grids[i]);
+                const int occ = std::max(1, kernel_occupancy(skind, kk[i], ldss[i]));
+                const int64_t busy = (nquads * strips[i] + kXcds - 1) / kXcds * kXcds;
+                grids[i] = (int)std::max<int64_t>(kXcds, std::min<int64_t>(occ * ex.num_cu / kXcds * kXcds, busy));
             } else {
                 grids[i] = grid_for(ex, kinds[i], kk[i], ldss[i], nquads);
-                bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+            }
+        }
+        auto is_split = [&](size_t i) {
+            const int bk = kinds[i] & ~kByteProfile;
+            return bk == kSplit || bk == kSplitF16;
+        };
+        // boundary rows and split bookkeeping (keys: 2 x count u64, done: 2 x
+        // quads u32, prog: slots u64, rel: slots u32) in 32-bit words
+        auto sizes = [&]() {
+            bnd_words = split_words = 0;
+            for (size_t i = 0; i < nb; ++i) {
+                const Bucket& bk = job.buckets[i];
+                if (is_split(i)) {
+                    bnd_words += (size_t)2 * grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+                    split_at[i] = split_words;
+                    split_words += 4 * (size_t)bk.count + 2 * (size_t)((bk.count + kPairsPerWave - 1) / kPairsPerWave) +
+                                   3 * (size_t)(2 * grids[i]);
+                } else {
+                    bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
+                }
+            }
+        };
+        sizes();
+        if (ex.fixed) {
+            // preallocated buffers (a serving slot): admission reserved the unsplit
+            // boundary rows; split buckets go back to the unsplit form, widest
+            // first, until everything fits
+            while (bnd_words * sizeof(uint32_t) > ex.bnd.bytes || split_words * sizeof(uint32_t) > ex.split_buf.bytes) {
+                size_t w = nb;
+                for (size_t i = 0; i < nb; ++i)
+                    if (is_split(i) && (w == nb || grids[i] > grids[w])) w = i;
+                if (w == nb) break;
+                kinds[w] = with_profile(batched_kind(job.buckets[w], max_sub, ex.f16 && !p8), p8);
+                grids[w] = grid_for(ex, kinds[w], kk[w], ldss[w],
+                                    (job.buckets[w].count + kPairsPerWave - 1) / kPairsPerWave);
+                sizes();
             }
         }
         if (split_words) {

@@ -81,6 +81,7 @@ struct Exec {
     int streams = kAuxStreams + 1; // streams for the length buckets (1 = serialised)
     bool f16 = true;               // exact f16 kernels where the score bound allows
     bool split = true;             // multi-strip buckets run their strips on several workgroups
+    bool fixed = false;            // buffers preallocated (serving slot): never grown by a job
     int kmodel = 1;                // length -> geometry model of new batches (kModel*)
     int init(int device, int num_cu);
     void destroy();
@@ -122,13 +123,18 @@ constexpr int kModelClassic = 0;
 constexpr int kModelOccupancy = 1;
 // K for a shorter sequence of `cols` residues.
 int k_for_cols(int cols, int alpha, int model);
-// Bucket ids: position of K in kernel_k_list, plus the list's length for pairs
-// that span several strips (bucket_ids() in all).
+// Bucket ids: position of K in kernel_k_list, plus once the list's length for
+// pairs that span several strips with at most kMultiShortCols columns, twice for
+// longer ones (bucket_ids() in all).
 int bucket_ids();
 // Host table shorter-length -> bucket id, 65536 entries, cached per (alpha, model).
 const std::vector<uint8_t>& bucket_table(int alpha, int model);
 // Key bits of the device radix sort: bucket id << 16 | (65535 - longer length).
-constexpr int kSortBits = 21;
+constexpr int kSortBits = 22;
+// Multi-strip pairs up to this many columns form buckets of their own (ids nk ..
+// 2nk-1), longer ones the ids 2nk .. 3nk-1: 512 x 3 + 3 < 2048 keeps DNA buckets
+// of the first class on the exact f16 kernels.
+constexpr int kMultiShortCols = 512;
 // Fill job.buckets from per-pair lengths (host), ordered by ascending K.
 void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry);
 // Workgroups of a full-occupancy batched launch for K columns per lane (the grid

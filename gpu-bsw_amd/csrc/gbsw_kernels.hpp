@@ -33,7 +33,7 @@ hipError_t zero_u32(uint32_t* p, int n, hipStream_t stream);
 // gbsw_binning.hip
 // Columns of one strip (32 x K) of each bucket id.
 struct StripWidths {
-    uint16_t w[32];
+    uint16_t w[64];
 };
 hipError_t bin_keys(const uint64_t* ref_off, const uint64_t* qry_off, int64_t n, const uint8_t* bucket_of_len,
                     const StripWidths& widths, uint32_t* keys, int32_t* vals, hipStream_t stream);

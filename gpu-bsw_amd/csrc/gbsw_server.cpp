@@ -47,7 +47,7 @@ double ms_since(Clock::time_point a, Clock::time_point b)
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-constexpr int kMaxBuckets = 32;
+constexpr int kMaxBuckets = 64;
 // Pinned host memory that kernels read in place: mapped, and coherent so that a
 // region rewritten by the host after an earlier read is never served stale.
 constexpr unsigned kDeviceReadFlags = hipHostMallocMapped | hipHostMallocCoherent;
@@ -316,7 +316,12 @@ int gbsw_server::init_slot(Slot& s)
     if (!rc) rc = alloc_into(s.sort_temp, s.sort_bytes, device_bytes);
     if (!rc) rc = alloc_into(s.ex.bnd, (size_t)cfg.boundary_bytes_per_slot, device_bytes);
     if (!rc) rc = alloc_into(s.ex.redo, 4 * P, device_bytes);
-    if (!rc) rc = alloc_into(s.ex.queue, 4 * kMaxBuckets, device_bytes);
+    if (!rc) rc = alloc_into(s.ex.queue, 4 * kXcds * kMaxBuckets, device_bytes);
+    // split-form bookkeeping: keys and counters of every pair, and the progress
+    // words of up to two boundary slots per workgroup of a full grid per bucket
+    const size_t split_words = 4 * P + 2 * (P / kPairsPerWave + kMaxBuckets) + 6 * (size_t)kMaxBuckets * 16 * s.ex.num_cu;
+    if (!rc) rc = alloc_into(s.ex.split_buf, 4 * split_words, device_bytes);
+    s.ex.fixed = true;
     if (rc) return rc;
     GBSW_HIP_TRY(s.meta_host.reserve(meta_bytes));
     GBSW_HIP_TRY(s.out_host.reserve(sizeof(int32_t) * 5 * P));
