@@ -237,8 +237,9 @@ def main():
     log("batched: %.1f GCUPS wall, %.1f GCUPS device, %.2f ms/step, %d launches" %
         (batched["gcups"], batched["device_gcups"], batched["ms_per_step"], len(batched["launches"])))
     for L in sorted(batched["launches"], key=lambda x: -x["ms"]):
-        log("   K=%2d  %7d pairs  %.3g cells  %7.3f ms  %7.1f GCUPS  grid %d" %
-            (L["columns_per_lane"], L["pairs"], L["cells"], L["ms"], L["cells"] / L["ms"] / 1e6, L["workgroups"]))
+        if L["ms"] > 0:  # (graph replays do not time single launches)
+            log("   K=%2d  %7d pairs  %.3g cells  %7.3f ms  %7.1f GCUPS  grid %d" %
+                (L["columns_per_lane"], L["pairs"], L["cells"], L["ms"], L["cells"] / L["ms"] / 1e6, L["workgroups"]))
     res_b = batch.results()
     plain = None
     if not args.no_plain:
@@ -257,22 +258,41 @@ def main():
     # duration there is the dominant kernel (one length bucket, persistent, forward
     # and reverse fused); its algorithmic work = 12 integer ops x the bucket's
     # forward cells (|ref| x |query|).
+    # (issued call by call: per-launch durations come from hipEvents around
+    # each launch, which a graph replay does not time)
     ctx.set_option(gbsw.OPT_STREAMS, 1)
+    ctx.set_option(gbsw.OPT_GRAPHS, 0)
     serial = time_mode(batch, scoring, "batched", 1, 1, cells)
     ctx.set_option(gbsw.OPT_STREAMS, 4)
-    dom = max(serial["launches"], key=lambda x: x["ms"])
-    kfn, kpol = KIND_NAMES.get(dom["kind"] & 0xFF, ("?", "?"))
-    kname = "%s<%s, %d, %s>" % (kfn, kpol, dom["columns_per_lane"],
-                                "true" if dom["kind"] & gbsw.KIND_BYTE_PROFILE else "false")
-    achieved = OPS_PER_CELL * dom["cells"] / (dom["ms"] / 1000.0) / 1e12
+    ctx.set_option(gbsw.OPT_GRAPHS, 1)
+    # The dominant kernel is the kernel (template instance) with the most time in
+    # the serialised step; one kernel may serve several buckets (a single-strip
+    # and a multi-strip one), so its duration is the average over its launches,
+    # as rocprofv3's kernel statistics give it.
+    per_kernel = {}
+    for L in serial["launches"]:
+        kfn, kpol = KIND_NAMES.get(L["kind"] & 0xFF, ("?", "?"))
+        name = "%s<%s, %d, %s>" % (kfn, kpol, L["columns_per_lane"],
+                                   "true" if L["kind"] & gbsw.KIND_BYTE_PROFILE else "false")
+        d = per_kernel.setdefault(name, {"ms": 0.0, "cells": 0, "pairs": 0, "launches": 0})
+        d["ms"] += L["ms"]
+        d["cells"] += L["cells"]
+        d["pairs"] += L["pairs"]
+        d["launches"] += 1
+    kname, dom = max(per_kernel.items(), key=lambda kv: kv[1]["ms"])
+    dom_ms = dom["ms"] / dom["launches"]
+    dom_cells = dom["cells"] / dom["launches"]
+    achieved = OPS_PER_CELL * dom_cells / (dom_ms / 1000.0) / 1e12
     agg = OPS_PER_CELL * cells / (batched["device_ms"] / 1000.0) / 1e12
-    log("dominant kernel (buckets serialised, %.2f ms/step): %s, %d pairs, %.4g cells, %.3f ms, %.1f GCUPS" %
-        (serial["device_ms"], kname, dom["pairs"], dom["cells"], dom["ms"], dom["cells"] / dom["ms"] / 1e6))
+    log("dominant kernel (buckets serialised, %.2f ms/step): %s, %d launch(es), %d pairs, %.4g cells per launch, "
+        "%.3f ms per launch, %.1f GCUPS" % (serial["device_ms"], kname, dom["launches"], dom["pairs"], dom_cells,
+                                           dom_ms, dom_cells / dom_ms / 1e6))
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(PEAK_INT_TOPS_PACKED, 1),
                 "unit": "TOP/s", "frac": round(achieved / PEAK_INT_TOPS_PACKED, 4),
                 "traffic": profiled_traffic(kname),
-                "kernel": "%s (bucket of %d pairs, %.4g cells)" % (kname, dom["pairs"], dom["cells"]),
-                "kernel_ms": round(dom["ms"], 3), "ops_per_cell": OPS_PER_CELL,
+                "kernel": "%s (%d launch(es) per step, %d pairs, %.4g cells per launch)" % (
+                    kname, dom["launches"], dom["pairs"], dom_cells),
+                "kernel_ms": round(dom_ms, 3), "ops_per_cell": OPS_PER_CELL,
                 "aggregate_achieved": round(agg, 3), "aggregate_frac": round(agg / PEAK_INT_TOPS_PACKED, 4)}
 
     batch.close()

@@ -44,6 +44,8 @@ struct gbsw_batch {
     uint8_t lut[256];
     gbsw_stats stats{};
     std::vector<LaunchRec> launches; // of the last gbsw_batch_align
+    JobGraph graph;
+    bool one_shot = false; // a gbsw_align batch: aligned once, never worth a graph
 };
 
 namespace gbsw {
@@ -126,6 +128,10 @@ int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value)
     case GBSW_OPT_F16:
         if (value != 0 && value != 1) return fail(GBSW_E_ARGUMENT, "f16 option is 0 or 1");
         ctx->ex.f16 = value != 0;
+        return GBSW_OK;
+    case GBSW_OPT_GRAPHS:
+        if (value != 0 && value != 1) return fail(GBSW_E_ARGUMENT, "graphs option is 0 or 1");
+        ctx->ex.graphs = value != 0;
         return GBSW_OK;
     case GBSW_OPT_SPLIT:
         if (value != 0 && value != 1) return fail(GBSW_E_ARGUMENT, "split option is 0 or 1");
@@ -272,6 +278,9 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
     b->stats = gbsw_stats{};
     b->stats.cells = b->job.cells;
     b->launches.clear();
+    // a resident batch captures its work into a graph; gbsw_align's one-off
+    // batch does not
+    b->job.graph = b->one_shot ? nullptr : &b->graph;
     int rc = enqueue_job(ctx->ex, b->job, sc, mode);
     if (rc) {
         (void)hipStreamSynchronize(ctx->ex.stream);
@@ -283,6 +292,7 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
     b->stats.device_ms = ms;
     b->stats.launches = (int32_t)b->launches.size();
     b->stats.buckets = mode == GBSW_PLAIN ? 1 : (int32_t)b->job.buckets.size();
+    b->stats.graph = ctx->ex.graph_state;
     for (const LaunchRec& L : b->launches) b->stats.main_kernel_ms = std::max(b->stats.main_kernel_ms, L.ms);
     return rc;
 }
@@ -336,6 +346,7 @@ int gbsw_align(gbsw_context* ctx, const gbsw_scoring* sc, int32_t mode, int64_t 
     gbsw_batch* b = nullptr;
     int rc = gbsw_batch_create(ctx, sc->alphabet, n, ref, ref_off, query, query_off, &b);
     if (rc) return rc;
+    b->one_shot = true;
     rc = gbsw_batch_align(b, sc, mode);
     if (!rc) rc = gbsw_batch_results(b, score, ref_begin, ref_end, query_begin, query_end);
     const std::string err = gbsw_last_error();

@@ -72,6 +72,12 @@ bool default_f16()
     return !(e && std::atoi(e) == 0);
 }
 
+bool default_graphs()
+{
+    const char* e = std::getenv("GBSW_GRAPHS");
+    return !(e && std::atoi(e) == 0);
+}
+
 bool default_split()
 {
     const char* e = std::getenv("GBSW_SPLIT");
@@ -155,6 +161,7 @@ int Exec::init(int dev, int ncu)
     streams = default_streams();
     f16 = default_f16();
     split = default_split();
+    graphs = default_graphs();
     kmodel = default_kmodel();
     hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
@@ -384,9 +391,190 @@ int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>
     return GBSW_OK;
 }
 
+namespace {
+
+// One length bucket's launch in the batched form.
+struct BucketPlan {
+    int kind = 0, K = 0, grid = 0, lds = 0, strips = 1;
+    size_t bnd_at = 0, split_at = 0; // 32-bit word offsets into ex.bnd / ex.split_buf
+};
+
+bool plan_is_split(const BucketPlan& b)
+{
+    const int k = b.kind & ~kByteProfile;
+    return k == kSplit || k == kSplitF16;
+}
+
+// Kernels, grids and scratch of every bucket (host only; reserves the buffers).
+int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int64_t bnd_rows,
+                 std::vector<BucketPlan>& plan, std::vector<size_t>& order)
+{
+    const size_t nb = job.buckets.size();
+    plan.assign(nb, BucketPlan{});
+    order.resize(nb);
+    for (size_t i = 0; i < nb; ++i) {
+        order[i] = i;
+        const Bucket& bk = job.buckets[i];
+        BucketPlan& b = plan[i];
+        b.K = std::min(kclamp, bk.K);
+        b.kind = with_profile(batched_kind(bk, max_sub, ex.f16 && !p8), p8);
+        b.lds = lds_for(b.K, job.alpha, p8);
+        const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
+        b.strips = std::max(1, (bk.max_cols + kVirtPerPair * b.K - 1) / (kVirtPerPair * b.K));
+        const int skind = with_profile((b.kind & ~kByteProfile) == kBatchedF16 ? kSplitF16 : kSplit, p8);
+        // A split item waits about four chunks for its predecessor's first
+        // rows: worth it only when the bucket has too few quads to keep every
+        // workgroup busy for two rounds (on the 200k-pair DNA mix: 1765 vs 827
+        // GCUPS at 1.4 quads per workgroup; 988 vs 1275 on protein at 2.7).
+        const int64_t full = (int64_t)std::max(1, kernel_occupancy(b.kind, b.K, b.lds)) * ex.num_cu;
+        if (ex.split && b.strips > 1 && nquads < 2 * full && kernel_has(skind, b.K)) {
+            // every XCD's counter needs workgroups: a multiple of kXcds, no more
+            // than the items a pipelined quad can keep busy, and two boundary
+            // slots per workgroup
+            b.kind = skind;
+            const int occ = std::max(1, kernel_occupancy(skind, b.K, b.lds));
+            const int64_t busy = (nquads * b.strips + kXcds - 1) / kXcds * kXcds;
+            b.grid = (int)std::max<int64_t>(kXcds, std::min<int64_t>(occ * ex.num_cu / kXcds * kXcds, busy));
+        } else {
+            b.grid = grid_for(ex, b.kind, b.K, b.lds, nquads);
+        }
+    }
+    // boundary rows and split bookkeeping (keys: 2 x count u64, done: 2 x quads
+    // u32, prog: slots u64, rel: slots u32) in 32-bit words
+    size_t bnd_words = 0, split_words = 0;
+    auto sizes = [&]() {
+        bnd_words = split_words = 0;
+        for (size_t i = 0; i < nb; ++i) {
+            const Bucket& bk = job.buckets[i];
+            BucketPlan& b = plan[i];
+            b.bnd_at = bnd_words;
+            if (plan_is_split(b)) {
+                bnd_words += (size_t)2 * b.grid * kPairsPerWave * (size_t)bnd_rows;
+                b.split_at = split_words;
+                split_words += 4 * (size_t)bk.count + 2 * (size_t)((bk.count + kPairsPerWave - 1) / kPairsPerWave) +
+                               3 * (size_t)(2 * b.grid);
+            } else {
+                bnd_words += (size_t)b.grid * kPairsPerWave * (size_t)bnd_rows;
+            }
+        }
+    };
+    sizes();
+    if (ex.fixed) {
+        // preallocated buffers (a serving slot): admission reserved the unsplit
+        // boundary rows; split buckets go back to the unsplit form, widest first,
+        // until everything fits
+        while (bnd_words * sizeof(uint32_t) > ex.bnd.bytes || split_words * sizeof(uint32_t) > ex.split_buf.bytes) {
+            size_t w = nb;
+            for (size_t i = 0; i < nb; ++i)
+                if (plan_is_split(plan[i]) && (w == nb || plan[i].grid > plan[w].grid)) w = i;
+            if (w == nb) break;
+            plan[w].kind = with_profile(batched_kind(job.buckets[w], max_sub, ex.f16 && !p8), p8);
+            plan[w].grid = grid_for(ex, plan[w].kind, plan[w].K, plan[w].lds,
+                                    (job.buckets[w].count + kPairsPerWave - 1) / kPairsPerWave);
+            sizes();
+        }
+    }
+    // Buckets run concurrently on 1 + kAuxStreams streams, the ones whose single
+    // work item takes longest (largest K) first, so a bucket with few long pairs
+    // overlaps the bulk instead of running alone at the end.
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return job.buckets[x].K > job.buckets[y].K; });
+    if (ex.queue.bytes < sizeof(uint32_t) * kXcds * std::max<size_t>(nb, 1))
+        GBSW_HIP_TRY(ex.queue.reserve(sizeof(uint32_t) * kXcds * std::max<size_t>(nb, 1)));
+    if (split_words) GBSW_HIP_TRY(ex.split_buf.reserve(split_words * sizeof(uint32_t)));
+    GBSW_HIP_TRY(ex.bnd.reserve(std::max<size_t>(bnd_words, 1) * sizeof(uint32_t)));
+    return GBSW_OK;
+}
+
+// Stream work of the batched form (recorded eagerly or into a graph).
+int record_batched(Exec& ex, const Job& job, KernelArgs a, int max_sub, const std::vector<BucketPlan>& plan,
+                   const std::vector<size_t>& order, size_t sub_bytes)
+{
+    hipStream_t st0 = ex.stream;
+    const uint8_t* th = ex.tables_host.as<uint8_t>();
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.sub.ptr, th, sub_bytes, hipMemcpyHostToDevice, st0));
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.lut.ptr, th + 2048, 256, hipMemcpyHostToDevice, st0));
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.colmax.ptr, th + 2304, sizeof(int16_t) * (size_t)job.alpha, hipMemcpyHostToDevice,
+                                st0));
+    GBSW_HIP_TRY(zero_u32(ex.status.as<uint32_t>(), kStatusWords, st0));
+    // device binning: keys, then a stable radix sort into the work order
+    GBSW_HIP_TRY(bin_keys(job.ref_off, job.qry_off, job.n, job.bucket_of_len, strip_widths(), job.keys, job.vals, st0));
+    size_t tb = job.sort_temp_bytes;
+    GBSW_HIP_TRY(sort_pairs(job.sort_temp, tb, job.keys, job.keys_sorted, job.vals, job.order, (int)job.n, kSortBits,
+                            st0));
+    const size_t nb = plan.size();
+    GBSW_HIP_TRY(zero_u32(ex.queue.as<uint32_t>(), (int)(kXcds * nb), st0));
+    size_t split_words = 0;
+    for (size_t i = 0; i < nb; ++i)
+        if (plan_is_split(plan[i]))
+            split_words = std::max(split_words, plan[i].split_at + 4 * (size_t)job.buckets[i].count +
+                                                    2 * (size_t)((job.buckets[i].count + kPairsPerWave - 1) / kPairsPerWave) +
+                                                    3 * (size_t)(2 * plan[i].grid));
+    if (split_words) GBSW_HIP_TRY(zero_u32(ex.split_buf.as<uint32_t>(), (int)split_words, st0));
+    const int ns = std::max(1, std::min(kAuxStreams + 1, ex.streams));
+    GBSW_HIP_TRY(hipEventRecord(ex.ev_fork, st0));
+    for (int i = 0; i < ns - 1; ++i) GBSW_HIP_TRY(hipStreamWaitEvent(ex.aux[i], ex.ev_fork, 0));
+    for (size_t j = 0; j < order.size(); ++j) {
+        const size_t i = order[j];
+        const Bucket& bk = job.buckets[i];
+        const BucketPlan& b = plan[i];
+        a.order = job.order + bk.begin;
+        a.order_n = bk.count;
+        a.queue = ex.queue.as<uint32_t>() + kXcds * i;
+        const int base_kind = b.kind & ~kByteProfile;
+        a.sat_limit = (base_kind == kBatchedF16 || base_kind == kSplitF16) ? kF16Exact - max_sub : 32767 - max_sub;
+        std::memset(&a.split, 0, sizeof(a.split));
+        if (plan_is_split(b)) {
+            uint32_t* w = ex.split_buf.as<uint32_t>() + b.split_at;
+            const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
+            const int64_t nslots = 2 * (int64_t)b.grid;
+            a.split.keys = (uint64_t*)w;
+            a.split.prog = (uint64_t*)(w + 4 * bk.count);
+            a.split.done = w + 4 * bk.count + 2 * nslots;
+            a.split.rel = a.split.done + 2 * nquads;
+            a.split.strips = b.strips;
+            a.split.slots = (int32_t)(nslots / kXcds);
+        }
+        hipStream_t st = (j % ns) == 0 ? st0 : ex.aux[j % ns - 1];
+        int rc = launch_timed(ex, st, b.kind, b.K, b.grid, b.lds, a, ex.bnd.as<uint32_t>() + b.bnd_at);
+        if (rc) return rc;
+        ex.launches.back().pairs = bk.count;
+        ex.launches.back().cells = bk.cells;
+    }
+    for (int i = 0; i < ns - 1; ++i) {
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_join[i], ex.aux[i]));
+        GBSW_HIP_TRY(hipStreamWaitEvent(st0, ex.ev_join[i], 0));
+    }
+    GBSW_HIP_TRY(hipMemcpyAsync(ex.status_host.ptr, ex.status.ptr, sizeof(int32_t) * kStatusWords,
+                                hipMemcpyDeviceToHost, st0));
+    return GBSW_OK;
+}
+
+// What a captured graph depends on besides the batch itself: the scoring, the
+// options, and the addresses of the context buffers it writes.
+std::string graph_key(const Exec& ex, const gbsw_scoring* sc, const std::vector<BucketPlan>& plan)
+{
+    // DNA reads the first four scores only
+    const size_t sc_bytes = sc->alphabet == GBSW_DNA ? offsetof(gbsw_scoring, matrix) : sizeof(*sc);
+    std::string k((const char*)sc, sc_bytes);
+    const void* ptrs[] = {ex.bnd.ptr,   ex.queue.ptr,          ex.split_buf.ptr, ex.redo.ptr,       ex.status.ptr,
+                          ex.sub.ptr,   ex.lut.ptr,            ex.colmax.ptr,    ex.tables_host.ptr, ex.status_host.ptr,
+                          ex.stream};
+    k.append((const char*)ptrs, sizeof(ptrs));
+    const int opts[] = {ex.streams, ex.f16 ? 1 : 0, ex.split ? 1 : 0, (int)plan.size()};
+    k.append((const char*)opts, sizeof(opts));
+    for (const BucketPlan& b : plan) {
+        const int64_t f[] = {b.kind, b.K, b.grid, b.lds, b.strips, (int64_t)b.bnd_at, (int64_t)b.split_at};
+        k.append((const char*)f, sizeof(f));
+    }
+    return k;
+}
+
+} // namespace
+
 int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
 {
     ex.launches.clear();
+    ex.graph_state = 0;
     std::vector<int16_t> sub;
     int max_sub = 0;
     bool p8 = false;
@@ -398,8 +586,8 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     // outside int8 take the int16 profile, in the widest geometry that fits
     const int kclamp = (byte_profile_alphabet(job.alpha) && !p8) ? max_k_for(job.alpha, false) : 1 << 30;
     hipStream_t st0 = ex.stream;
-    GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st0));
     if (job.n == 0) {
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st0));
         GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st0));
         std::memset(ex.status_host.ptr, 0, sizeof(int32_t) * kStatusWords);
         return GBSW_OK;
@@ -414,13 +602,14 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     std::memcpy(th, sub.data(), sizeof(int16_t) * sub.size());
     std::memcpy(th + 2048, job.lut_host, 256);
     std::memcpy(th + 2304, colmax.data(), sizeof(int16_t) * colmax.size());
-    GBSW_HIP_TRY(hipMemcpyAsync(ex.sub.ptr, th, sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, st0));
-    GBSW_HIP_TRY(hipMemcpyAsync(ex.lut.ptr, th + 2048, 256, hipMemcpyHostToDevice, st0));
-    GBSW_HIP_TRY(hipMemcpyAsync(ex.colmax.ptr, th + 2304, sizeof(int16_t) * colmax.size(), hipMemcpyHostToDevice, st0));
     GBSW_HIP_TRY(ex.redo.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(1, job.n)));
     KernelArgs a = base_args(ex, job, sc);
-    GBSW_HIP_TRY(zero_u32(ex.status.as<uint32_t>(), kStatusWords, st0));
     if (mode == GBSW_PLAIN) {
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st0));
+        GBSW_HIP_TRY(hipMemcpyAsync(ex.sub.ptr, th, sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, st0));
+        GBSW_HIP_TRY(hipMemcpyAsync(ex.lut.ptr, th + 2048, 256, hipMemcpyHostToDevice, st0));
+        GBSW_HIP_TRY(hipMemcpyAsync(ex.colmax.ptr, th + 2304, sizeof(int16_t) * colmax.size(), hipMemcpyHostToDevice, st0));
+        GBSW_HIP_TRY(zero_u32(ex.status.as<uint32_t>(), kStatusWords, st0));
         // one geometry for the whole batch, sized by its longest shorter sequence;
         // with profiles (DNA) the occupancy model's narrower strips cost the long
         // pairs more than the extra waves gain (793 vs 593 GCUPS on the 1M-pair
@@ -442,134 +631,57 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         rc = launch_timed(ex, st0, kr, K, gr, lds, a, ex.bnd.ptr);
         if (rc) return rc;
         ex.launches.back().pairs = job.n;
-    } else {
-        // device binning: keys, then a stable radix sort into the work order
-        GBSW_HIP_TRY(bin_keys(job.ref_off, job.qry_off, job.n, job.bucket_of_len, strip_widths(), job.keys, job.vals,
-                              st0));
-        size_t tb = job.sort_temp_bytes;
-        GBSW_HIP_TRY(sort_pairs(job.sort_temp, tb, job.keys, job.keys_sorted, job.vals, job.order, (int)job.n,
-                                kSortBits, st0));
-        const size_t nb = job.buckets.size();
-        // kXcds work counters per bucket (k_batched uses the first)
-        if (ex.queue.bytes < sizeof(uint32_t) * kXcds * nb) GBSW_HIP_TRY(ex.queue.reserve(sizeof(uint32_t) * kXcds * nb));
-        GBSW_HIP_TRY(zero_u32(ex.queue.as<uint32_t>(), (int)(kXcds * nb), st0));
-        // Buckets run concurrently on 1 + kAuxStreams streams, the ones whose single
-        // work item takes longest (largest K) first, so a bucket with few long
-        // pairs overlaps the bulk instead of running alone at the end.
-        std::vector<size_t> order(nb);
-        std::vector<int> grids(nb), ldss(nb), kinds(nb), kk(nb), strips(nb, 1);
-        std::vector<size_t> split_at(nb, 0);
-        size_t bnd_words = 0, split_words = 0;
-        for (size_t i = 0; i < nb; ++i) {
-            order[i] = i;
-            const Bucket& bk = job.buckets[i];
-            kk[i] = std::min(kclamp, bk.K);
-            kinds[i] = with_profile(batched_kind(bk, max_sub, ex.f16 && !p8), p8);
-            ldss[i] = lds_for(kk[i], job.alpha, p8);
-            const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
-            strips[i] = std::max(1, (bk.max_cols + kVirtPerPair * kk[i] - 1) / (kVirtPerPair * kk[i]));
-            const int skind = with_profile((kinds[i] & ~kByteProfile) == kBatchedF16 ? kSplitF16 : kSplit, p8);
-            // A split item waits about four chunks for its predecessor's first
-            // rows: worth it only when the bucket has too few quads to keep every
-            // workgroup busy for two rounds (on the 200k-pair DNA mix: 1765 vs 827
-            // GCUPS at 1.4 quads per workgroup; 988 vs 1275 on protein at 2.7).
-            const int64_t full = (int64_t)std::max(1, kernel_occupancy(kinds[i], kk[i], ldss[i])) * ex.num_cu;
-            if (ex.split && strips[i] > 1 && nquads < 2 * full && kernel_has(skind, kk[i])) {
-                // every XCD's counter needs workgroups: a multiple of kXcds, no
-                // more than the items a pipelined quad can keep busy, and two
-                // boundary slots per workgroup
-                kinds[i] = skind;
-                const int occ = std::max(1, kernel_occupancy(skind, kk[i], ldss[i]));
-                const int64_t busy = (nquads * strips[i] + kXcds - 1) / kXcds * kXcds;
-                grids[i] = (int)std::max<int64_t>(kXcds, std::min<int64_t>(occ * ex.num_cu / kXcds * kXcds, busy));
-            } else {
-                grids[i] = grid_for(ex, kinds[i], kk[i], ldss[i], nquads);
-            }
-        }
-        auto is_split = [&](size_t i) {
-            const int bk = kinds[i] & ~kByteProfile;
-            return bk == kSplit || bk == kSplitF16;
-        };
-        // boundary rows and split bookkeeping (keys: 2 x count u64, done: 2 x
-        // quads u32, prog: slots u64, rel: slots u32) in 32-bit words
-        auto sizes = [&]() {
-            bnd_words = split_words = 0;
-            for (size_t i = 0; i < nb; ++i) {
-                const Bucket& bk = job.buckets[i];
-                if (is_split(i)) {
-                    bnd_words += (size_t)2 * grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
-                    split_at[i] = split_words;
-                    split_words += 4 * (size_t)bk.count + 2 * (size_t)((bk.count + kPairsPerWave - 1) / kPairsPerWave) +
-                                   3 * (size_t)(2 * grids[i]);
-                } else {
-                    bnd_words += (size_t)grids[i] * kPairsPerWave * (size_t)a.bnd_rows;
-                }
-            }
-        };
-        sizes();
-        if (ex.fixed) {
-            // preallocated buffers (a serving slot): admission reserved the unsplit
-            // boundary rows; split buckets go back to the unsplit form, widest
-            // first, until everything fits
-            while (bnd_words * sizeof(uint32_t) > ex.bnd.bytes || split_words * sizeof(uint32_t) > ex.split_buf.bytes) {
-                size_t w = nb;
-                for (size_t i = 0; i < nb; ++i)
-                    if (is_split(i) && (w == nb || grids[i] > grids[w])) w = i;
-                if (w == nb) break;
-                kinds[w] = with_profile(batched_kind(job.buckets[w], max_sub, ex.f16 && !p8), p8);
-                grids[w] = grid_for(ex, kinds[w], kk[w], ldss[w],
-                                    (job.buckets[w].count + kPairsPerWave - 1) / kPairsPerWave);
-                sizes();
-            }
-        }
-        if (split_words) {
-            GBSW_HIP_TRY(ex.split_buf.reserve(split_words * sizeof(uint32_t)));
-            GBSW_HIP_TRY(zero_u32(ex.split_buf.as<uint32_t>(), (int)split_words, st0));
-        }
-        std::sort(order.begin(), order.end(),
-                  [&](size_t x, size_t y) { return job.buckets[x].K > job.buckets[y].K; });
-        GBSW_HIP_TRY(ex.bnd.reserve(std::max<size_t>(bnd_words, 1) * sizeof(uint32_t)));
-        const int ns = std::max(1, std::min(kAuxStreams + 1, ex.streams));
-        GBSW_HIP_TRY(hipEventRecord(ex.ev_fork, st0));
-        for (int i = 0; i < ns - 1; ++i) GBSW_HIP_TRY(hipStreamWaitEvent(ex.aux[i], ex.ev_fork, 0));
-        size_t bnd_at = 0;
-        for (size_t j = 0; j < order.size(); ++j) {
-            const size_t i = order[j];
-            const Bucket& bk = job.buckets[i];
-            a.order = job.order + bk.begin;
-            a.order_n = bk.count;
-            a.queue = ex.queue.as<uint32_t>() + kXcds * i;
-            const int base_kind = kinds[i] & ~kByteProfile;
-            a.sat_limit = (base_kind == kBatchedF16 || base_kind == kSplitF16) ? kF16Exact - max_sub : 32767 - max_sub;
-            std::memset(&a.split, 0, sizeof(a.split));
-            if (base_kind == kSplit || base_kind == kSplitF16) {
-                uint32_t* w = ex.split_buf.as<uint32_t>() + split_at[i];
-                const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
-                const int64_t nslots = 2 * (int64_t)grids[i];
-                a.split.keys = (uint64_t*)w;
-                a.split.prog = (uint64_t*)(w + 4 * bk.count);
-                a.split.done = w + 4 * bk.count + 2 * nslots;
-                a.split.rel = a.split.done + 2 * nquads;
-                a.split.strips = strips[i];
-                a.split.slots = (int32_t)(nslots / kXcds);
-            }
-            hipStream_t st = (j % ns) == 0 ? st0 : ex.aux[j % ns - 1];
-            rc = launch_timed(ex, st, kinds[i], kk[i], grids[i], ldss[i], a, ex.bnd.as<uint32_t>() + bnd_at);
-            if (rc) return rc;
-            ex.launches.back().pairs = bk.count;
-            ex.launches.back().cells = bk.cells;
-            const int base_kind2 = kinds[i] & ~kByteProfile;
-            bnd_at += (size_t)((base_kind2 == kSplit || base_kind2 == kSplitF16) ? 2 : 1) * grids[i] * kPairsPerWave *
-                      (size_t)a.bnd_rows;
-        }
-        for (int i = 0; i < ns - 1; ++i) {
-            GBSW_HIP_TRY(hipEventRecord(ex.ev_join[i], ex.aux[i]));
-            GBSW_HIP_TRY(hipStreamWaitEvent(st0, ex.ev_join[i], 0));
-        }
+        GBSW_HIP_TRY(hipMemcpyAsync(ex.status_host.ptr, ex.status.ptr, sizeof(int32_t) * kStatusWords,
+                                    hipMemcpyDeviceToHost, st0));
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st0));
+        return GBSW_OK;
     }
-    GBSW_HIP_TRY(hipMemcpyAsync(ex.status_host.ptr, ex.status.ptr, sizeof(int32_t) * kStatusWords,
-                                hipMemcpyDeviceToHost, st0));
+    std::vector<BucketPlan> plan;
+    std::vector<size_t> order;
+    rc = plan_batched(ex, job, max_sub, p8, kclamp, a.bnd_rows, plan, order);
+    if (rc) return rc;
+    const size_t sub_bytes = sizeof(int16_t) * sub.size();
+    if (!ex.graphs || ex.fixed || !job.graph) {
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st0));
+        rc = record_batched(ex, job, a, max_sub, plan, order, sub_bytes);
+        if (rc) return rc;
+        GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st0));
+        return GBSW_OK;
+    }
+    // hipGraph: the stream work of this batch, scoring and options is captured
+    // once and replayed (one launch instead of ~20 API calls); the per-launch
+    // records of the capture stand for every replay
+    JobGraph& G = *job.graph;
+    const std::string key = graph_key(ex, sc, plan);
+    if (!G.exec || G.key != key) {
+        G.reset();
+        GBSW_HIP_TRY(hipStreamBeginCapture(st0, hipStreamCaptureModeRelaxed));
+        rc = record_batched(ex, job, a, max_sub, plan, order, sub_bytes);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(st0, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(GBSW_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+        const hipError_t ie = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            G.exec = nullptr;
+            return fail(GBSW_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+        }
+        G.key = key;
+        G.launches = ex.launches;
+        G.captures += 1;
+        ex.graph_state = 1;
+    } else {
+        ex.graph_state = 2;
+    }
+    ex.launches = G.launches;
+    GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st0));
+    GBSW_HIP_TRY(hipGraphLaunch(G.exec, st0));
     GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st0));
+    G.replays += 1;
     return GBSW_OK;
 }
 
@@ -608,8 +720,12 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
     GBSW_HIP_TRY(hipEventElapsedTime(&ms, ex.ev_start, ex.ev_stop));
     if (device_ms) *device_ms = ms;
     for (size_t i = 0; i < ex.launches.size(); ++i) {
+        // events recorded by a graph replay may not carry times: 0 then
         float t = 0.f;
-        GBSW_HIP_TRY(hipEventElapsedTime(&t, ex.kev[2 * i], ex.kev[2 * i + 1]));
+        if (hipEventElapsedTime(&t, ex.kev[2 * i], ex.kev[2 * i + 1]) != hipSuccess) {
+            t = 0.f;
+            (void)hipGetLastError();
+        }
         ex.launches[i].ms = t;
     }
     if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");

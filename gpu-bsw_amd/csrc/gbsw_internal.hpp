@@ -82,13 +82,34 @@ struct Exec {
     bool f16 = true;               // exact f16 kernels where the score bound allows
     bool split = true;             // multi-strip buckets run their strips on several workgroups
     bool fixed = false;            // buffers preallocated (serving slot): never grown by a job
+    bool graphs = true;            // batched form of a resident batch replays a captured hipGraph
+    int graph_state = 0;           // last job: 0 eager, 1 captured, 2 replayed
     int kmodel = 1;                // length -> geometry model of new batches (kModel*)
     int init(int device, int num_cu);
     void destroy();
 };
 
 // Device view of the pairs of one job (a resident batch, or a serving slot).
+// The captured stream work of a batch's batched form, with what it depends on.
+struct JobGraph {
+    hipGraphExec_t exec = nullptr;
+    std::string key;
+    std::vector<LaunchRec> launches; // records of the capture
+    int64_t captures = 0, replays = 0;
+    void reset()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    JobGraph() = default;
+    JobGraph(const JobGraph&) = delete;
+    JobGraph& operator=(const JobGraph&) = delete;
+    ~JobGraph() { reset(); }
+};
+
 struct Job {
+    JobGraph* graph = nullptr;       // set for resident batches: graph capture allowed
     int32_t alphabet = GBSW_DNA;
     int64_t n = 0;
     const uint8_t* ref = nullptr;

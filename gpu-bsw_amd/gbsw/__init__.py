@@ -53,6 +53,8 @@ class Stats(ctypes.Structure):
         ("cells", ctypes.c_int64),
         ("launches", ctypes.c_int32),
         ("buckets", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -112,7 +114,7 @@ EXPORTED = (
     "gbsw_server_wait", "gbsw_server_poll", "gbsw_server_stats", "gbsw_host_alloc", "gbsw_host_free",
     "gbsw_context_set_option",
 )
-OPT_STREAMS, OPT_F16, OPT_GEOMETRY, OPT_SPLIT = 1, 2, 3, 4
+OPT_STREAMS, OPT_F16, OPT_GEOMETRY, OPT_SPLIT, OPT_GRAPHS = 1, 2, 3, 4, 5
 # launch kinds (gbsw_launch_stat.kind): plain forward, plain reverse, batched
 # int16, batched int32, batched f16; plus KIND_BYTE_PROFILE for int8 profiles
 KIND_PLAIN_FORWARD, KIND_PLAIN_REVERSE, KIND_BATCHED, KIND_BATCHED_WIDE, KIND_BATCHED_F16 = 0, 1, 2, 3, 4
@@ -221,7 +223,7 @@ class Context:
             pass
 
     def set_option(self, option, value):
-        """OPT_STREAMS (1..4), OPT_F16 (0/1), OPT_GEOMETRY (0/1) or OPT_SPLIT (0/1); see include/gbsw.h."""
+        """OPT_STREAMS (1..4), OPT_F16, OPT_GEOMETRY, OPT_SPLIT or OPT_GRAPHS (0/1); see include/gbsw.h."""
         _check(self._lib.gbsw_context_set_option(self._h, option, int(value)))
 
     def batch(self, alphabet, ref, ref_off, query, query_off):

@@ -66,6 +66,9 @@ typedef struct {
     int64_t cells;          /* sum of |ref|*|query| (the reference's "Total Cells") */
     int32_t launches;       /* kernel launches issued                               */
     int32_t buckets;        /* length buckets used (BATCHED)                        */
+    int32_t graph;          /* 0 issued call by call, 1 captured into a hipGraph
+                               and launched, 2 a replay of the captured graph       */
+    int32_t reserved;
 } gbsw_stats;
 
 /* Create a context on one HIP device.  A context owns a stream, scratch buffers
@@ -97,6 +100,15 @@ int gbsw_context_destroy(gbsw_context* ctx);
  *          before as they are written); 0: one workgroup runs a pair's strips
  *          one after another.  Results are identical. */
 #define GBSW_OPT_SPLIT 4
+/* GRAPHS:  1 (default; GBSW_GRAPHS=0 turns it off): gbsw_batch_align in the
+ *          batched form captures its stream work (table copies, binning, sort,
+ *          every bucket launch on every stream) into a hipGraph the first time,
+ *          and replays it while the batch, scoring, options and context buffers
+ *          stay the same (gbsw_align, which aligns its batch once, does not
+ *          capture); 0: the work is issued call by call.  Per-launch durations
+ *          (gbsw_batch_launch_stats) may be 0 for graph launches.  Results are
+ *          identical. */
+#define GBSW_OPT_GRAPHS 5
 int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value);
 
 /* Message of the last error on this thread ("" if none). */

@@ -296,6 +296,46 @@ def test_batched_options_give_identical_results(ctx):
         ctx.set_option(gbsw.OPT_STREAMS, 9)
 
 
+def test_graph_replays_match_eager_results(ctx):
+    """A resident batch replays the hipGraph captured at its first align: every replay,
+    with eager aligns of other data and synchronisations between the replays,
+    equals the eager result; a new scoring captures again, and going back to the
+    first scoring captures again too (the graph is keyed by scoring and options)."""
+    w = synth.batched_dna(3000, seed=0x5EED0208)
+    wp = synth.plain_protein(500, seed=0x5EED0209)
+    refs, qrys = rand_pairs(22, 60, 600, 1800, ["ACGT"], 300, 1100)   # multi-strip, split form
+    rb, ro = gbsw.pack(refs)
+    qb, qo = gbsw.pack(qrys)
+    sc1, sc2 = gbsw.dna_scoring(*w.scoring), gbsw.dna_scoring(2, -3, -5, -2)
+    exp1 = oracle_results(w.ref, w.ref_off, w.query, w.query_off, gbsw.DNA, w.scoring)
+    exp2 = oracle_results(w.ref, w.ref_off, w.query, w.query_off, gbsw.DNA, (2, -3, -5, -2))
+    expm = oracle_results(rb, ro, qb, qo, gbsw.DNA, w.scoring)
+    expp = oracle_results(wp.ref, wp.ref_off, wp.query, wp.query_off, gbsw.PROTEIN, wp.scoring)
+    a = ctx.batch(gbsw.DNA, w.ref, w.ref_off, w.query, w.query_off)
+    m = ctx.batch(gbsw.DNA, rb, ro, qb, qo)
+    try:
+        states = []
+        for rep, (sc, exp) in enumerate([(sc1, exp1)] * 4 + [(sc2, exp2)] * 3 + [(sc1, exp1)] * 2):
+            print("case graph replay", rep, flush=True)
+            a.align(sc, "batched")
+            states.append(a.stats()["graph"])
+            assert_same(a.results(), exp, "graph replay %d" % rep)
+            # eager and graphed work of other batches between the replays
+            got = ctx.align_packed(wp.ref, wp.ref_off, wp.query, wp.query_off, gbsw_scoring(gbsw.PROTEIN, wp.scoring),
+                                   "batched")
+            assert_same(got, expp, "eager protein between replays %d" % rep)
+            m.align(sc1, "batched")
+            assert_same(m.results(), expm, "multi-strip batch between replays %d" % rep)
+        # captured at the first align (again if another batch grew the context's
+        # buffers meanwhile) and replayed after; each scoring change captures again
+        assert states[0] == 1 and states[4] == 1 and states[7] == 1, states
+        assert states[2:4] == [2, 2] and states[5:7] == [2, 2] and states[8] == 2, states
+        assert m.stats()["graph"] == 2
+    finally:
+        a.close()
+        m.close()
+
+
 def test_geometry_models_give_identical_results(ctx):
     """Both length -> geometry models (the plain form of protein follows the
     option too): the same results."""
