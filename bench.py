@@ -202,6 +202,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the 10k DNA / 20k protein / long-protein lines")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default length -> geometry map")
     ap.add_argument("--long-pairs", type=int, default=50_000, help="pairs of the long-protein set")
     ap.add_argument("--serving-window", type=float, default=5.0, help="seconds per offered load")
     ap.add_argument("--serving-loads", default="0.25,0.5,0.75,1.0")
@@ -232,6 +233,13 @@ def main():
     ctx = gbsw.Context(0)
     batch = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
     scoring = gbsw.dna_scoring(*w.scoring)
+    tuned = None
+    if not args.no_autotune:
+        # untimed set-up: the length -> geometry map measured on this batch
+        t = time.perf_counter()
+        model, model_ms = batch.autotune(scoring, repeats=3)
+        tuned = {"geometry_model": model, "device_ms": round(model_ms, 3), "tuning_s": round(time.perf_counter() - t, 2)}
+        log("autotune: geometry model %d (%.2f ms device), %.1f s" % (model, model_ms, tuned["tuning_s"]))
 
     batched = time_mode(batch, scoring, "batched", args.warmup, args.steps, cells)
     log("batched: %.1f GCUPS wall, %.1f GCUPS device, %.2f ms/step, %d launches" %
@@ -331,7 +339,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "batched uneven-length DNA mix: %d seeded pairs, query lognormal(median 150) "
                                "[32,1024], ref lognormal(median 600) [64,2048], +3/-3/-6/-1" % w.n,
-                   "pairs": w.n, "cells": cells, "form": "batched"},
+                   "pairs": w.n, "cells": cells, "form": "batched", "autotune": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device_gcups": round(batched["device_gcups"], 2),

@@ -138,7 +138,7 @@ int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value)
         ctx->ex.split = value != 0;
         return GBSW_OK;
     case GBSW_OPT_GEOMETRY:
-        if (value != kModelClassic && value != kModelOccupancy) return fail(GBSW_E_ARGUMENT, "geometry model is 0 or 1");
+        if (value < 0 || value >= kGeometryModels) return fail(GBSW_E_ARGUMENT, "geometry model is 0..5");
         ctx->ex.kmodel = (int)value;
         return GBSW_OK;
     default:
@@ -335,6 +335,51 @@ int gbsw_batch_launch_stats(const gbsw_batch* b, int32_t index, gbsw_launch_stat
     out->pairs = L.pairs;
     out->cells = L.cells;
     out->ms = L.ms;
+    return GBSW_OK;
+}
+
+int gbsw_batch_autotune(gbsw_batch* b, const gbsw_scoring* sc, int32_t repeats, int32_t* model, double* ms)
+{
+    if (!b || !sc) return fail(GBSW_E_ARGUMENT, "null argument");
+    if (repeats < 1) repeats = 1;
+    gbsw_context* ctx = b->ctx;
+    GBSW_HIP_TRY(hipSetDevice(ctx->device));
+    Job& job = b->job;
+    auto use_model = [&](int m) -> int {
+        GBSW_HIP_TRY(hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, m).data(), 65536, hipMemcpyHostToDevice));
+        job.kmodel = m;
+        count_buckets(job, b->len_ref.data(), b->len_qry.data());
+        return GBSW_OK;
+    };
+    // the batch's current model first; another one must beat it by more than
+    // kTuneMargin (run-to-run noise of a batch is about 1 %)
+    constexpr double kTuneMargin = 0.015;
+    const int current = job.kmodel;
+    int best = -1;
+    double best_ms = 1e300;
+    std::vector<const std::vector<uint8_t>*> tried;
+    for (int i = 0; i < kGeometryModels; ++i) {
+        const int m = i == 0 ? current : (i <= current ? i - 1 : i);
+        // models whose length -> geometry map is one already timed add nothing
+        const std::vector<uint8_t>* t = &bucket_table(job.alpha, m);
+        bool seen = false;
+        for (const auto* u : tried) seen = seen || *u == *t;
+        if (seen) continue;
+        tried.push_back(t);
+        int rc = use_model(m);
+        double m_ms = 1e300;
+        for (int r = 0; !rc && r < repeats + 1; ++r) {
+            rc = gbsw_batch_align(b, sc, GBSW_BATCHED);
+            if (!rc && r > 0) m_ms = std::min(m_ms, b->stats.device_ms); // the first run captures the graph
+        }
+        if (rc) return rc;
+        if (best < 0 || m_ms < best_ms * (1.0 - kTuneMargin)) { best_ms = m_ms; best = m; }
+    }
+    int rc = use_model(best);
+    if (rc) return rc;
+    ctx->ex.kmodel = best; // batches created afterwards start from it
+    if (model) *model = best;
+    if (ms) *ms = best_ms;
     return GBSW_OK;
 }
 

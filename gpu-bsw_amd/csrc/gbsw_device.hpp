@@ -33,8 +33,8 @@ __device__ __forceinline__ uint32_t pk_sub_floor0(uint32_t a, uint32_t b)
     return raw(__builtin_elementwise_sub_sat(as_u(a), as_u(b)));
 }
 // 0xFFFF in each half where a < b (signed; |a - b| < 2^15), else 0:
-// v_pk_sub_i16 + v_pk_ashrrev_i16.  Written as asm because the compiler
-// otherwise expands the sign test into compares, selects and a permute.
+// v_pk_sub_i16 + v_pk_ashrrev_i16.  Written as asm: in the int16 kernels the
+// compiler's own form of this sign test measured 3 % slower.
 __device__ __forceinline__ uint32_t pk_lt_mask(uint32_t a, uint32_t b)
 {
     uint32_t d, m;
@@ -55,29 +55,23 @@ __device__ __forceinline__ f16x2 as_h(uint32_t v) { return __builtin_bit_cast(f1
 __device__ __forceinline__ uint32_t raw(f16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 // v_pk_add_f16
 __device__ __forceinline__ uint32_t pk_add_f16(uint32_t a, uint32_t b) { return raw(as_h(a) + as_h(b)); }
-// v_pk_max_f16.  Written as asm: the operands are integers (never NaN), and the
-// builtin would first canonicalise each operand with an extra max.
+// IEEE maximum of both halves (llvm.maximum: v_pk_maximum3_f16 on gfx950, with no
+// canonicalising max per operand, unlike maxnum).  The operands are integers,
+// never NaN, and -0 does not arise (a sum is -0 only if both terms are).
 __device__ __forceinline__ uint32_t pk_max_f16(uint32_t a, uint32_t b)
 {
-    uint32_t r;
-    asm("v_pk_max_f16 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
+    return raw(__builtin_elementwise_maximum(as_h(a), as_h(b)));
 }
 // v_pk_maximum3_f16 (gfx950): max of three per half.
 __device__ __forceinline__ uint32_t pk_max3_f16(uint32_t a, uint32_t b, uint32_t c)
 {
-    uint32_t r;
-    asm("v_pk_maximum3_f16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
+    return raw(__builtin_elementwise_maximum(__builtin_elementwise_maximum(as_h(a), as_h(b)), as_h(c)));
 }
 // 0xFFFF in each half where a < b, else 0: the sign of a - b (exact, and +0 when
 // equal), spread by v_pk_ashrrev_i16.
 __device__ __forceinline__ uint32_t pk_lt_mask_f16(uint32_t a, uint32_t b)
 {
-    uint32_t d, m;
-    asm("v_pk_add_f16 %0, %1, %2 neg_lo:[0,1] neg_hi:[0,1]" : "=v"(d) : "v"(a), "v"(b));
-    asm("v_pk_ashrrev_i16 %0, 15, %1 op_sel_hi:[0,1]" : "=v"(m) : "v"(d));
-    return m;
+    return raw(as_s(raw(as_h(a) - as_h(b))) >> (s16x2){15, 15});
 }
 __host__ __device__ inline uint16_t f16_bits(int32_t x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 __device__ __forceinline__ int32_t f16_int(uint32_t bits16)

@@ -87,7 +87,8 @@ bool default_split()
 int default_kmodel()
 {
     const char* e = std::getenv("GBSW_KMODEL");
-    return (e && std::atoi(e) == kModelClassic) ? kModelClassic : kModelOccupancy;
+    const int m = e ? std::atoi(e) : kModelOccupancy;
+    return (m >= 0 && m < kGeometryModels) ? m : kModelOccupancy;
 }
 
 int default_streams()
@@ -217,23 +218,34 @@ int max_k_for(int alpha, bool p8)
     return best;
 }
 
+double occupancy_threshold(int model)
+{
+    static const double thr[kGeometryModels] = {0.0, 2.5, 2.0, 3.0, 4.0, 1.5};
+    return (model > 0 && model < kGeometryModels) ? thr[model] : 0.0;
+}
+
 // Share of the VALU issue rate a geometry reaches at its occupancy: the large
 // DNA buckets issue 0.96 VALU instructions per CU per cycle at 4 waves per SIMD,
 // 0.82 at 2.75, 0.78 at 2 and 0.62 at 1.5 (rocprofv3 counters), so
-// min(1, waves per SIMD / 2.5).
-double occupancy_factor(int K, int alpha)
+// min(1, waves per SIMD / threshold), threshold 2.5 by default.
+double occupancy_factor(int K, int alpha, double threshold)
 {
     static std::mutex mu;
-    static std::map<int, double> cache;
-    std::lock_guard<std::mutex> lk(mu);
-    const int key = K * 64 + alpha;
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    const bool p8 = byte_profile_alphabet(alpha);
-    const int wg_per_cu = kernel_occupancy(with_profile(kBatched, p8), K, lds_for(K, alpha, p8)); // one wave each
-    const double f = wg_per_cu > 0 ? std::min(1.0, wg_per_cu / 4.0 / 2.5) : 1.0;
-    cache.emplace(key, f);
-    return f;
+    static std::map<int, int> cache; // workgroups per CU
+    int wg_per_cu = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        const int key = K * 64 + alpha;
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            wg_per_cu = it->second;
+        } else {
+            const bool p8 = byte_profile_alphabet(alpha);
+            wg_per_cu = kernel_occupancy(with_profile(kBatched, p8), K, lds_for(K, alpha, p8)); // one wave each
+            cache.emplace(key, wg_per_cu);
+        }
+    }
+    return (wg_per_cu > 0 && threshold > 0) ? std::min(1.0, wg_per_cu / 4.0 / threshold) : 1.0;
 }
 
 int k_for_cols(int cols, int alpha, int model)
@@ -249,7 +261,7 @@ int k_for_cols(int cols, int alpha, int model)
         const int64_t strips = std::max<int64_t>(1, (cols + kVirtPerPair * K - 1) / (kVirtPerPair * K));
         // padded columns and the per-step cost, per strip
         double cost = (double)(strips * (K + 2) * 4 + (strips - 1));
-        if (model == kModelOccupancy) cost /= occupancy_factor(K, alpha);
+        if (model != kModelClassic) cost /= occupancy_factor(K, alpha, occupancy_threshold(model));
         if (cost < best_cost) { best_cost = cost; best = K; }
     }
     return best;
@@ -277,7 +289,7 @@ const std::vector<uint8_t>& bucket_table(int alpha, int model)
     static std::mutex mu;
     static std::map<int, std::vector<uint8_t>> cache;
     std::lock_guard<std::mutex> lk(mu);
-    const int key = alpha * 4 + model;
+    const int key = alpha * 64 + model;
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int nk = 0;

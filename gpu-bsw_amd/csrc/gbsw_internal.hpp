@@ -142,6 +142,10 @@ inline bool byte_profile_alphabet(int alpha) { return alpha >= kByteProfileAlpha
 // the VALU issue share the geometry's occupancy allows.
 constexpr int kModelClassic = 0;
 constexpr int kModelOccupancy = 1;
+// models 1.. divide the cost by min(1, waves per SIMD / threshold), threshold
+// 2.5, 2.0, 3.0, 4.0, 1.5 waves
+constexpr int kGeometryModels = 6;
+double occupancy_threshold(int model);
 // K for a shorter sequence of `cols` residues.
 int k_for_cols(int cols, int alpha, int model);
 // Bucket ids: position of K in kernel_k_list, plus once the list's length for

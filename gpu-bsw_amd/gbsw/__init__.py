@@ -109,7 +109,8 @@ class ServingStats(ctypes.Structure):
 EXPORTED = (
     "gbsw_context_create", "gbsw_context_destroy", "gbsw_last_error", "gbsw_batch_create",
     "gbsw_batch_destroy", "gbsw_batch_align", "gbsw_batch_results", "gbsw_batch_device_results",
-    "gbsw_batch_stats", "gbsw_batch_launch_stats", "gbsw_align", "gbsw_device_memory", "gbsw_version",
+    "gbsw_batch_stats", "gbsw_batch_launch_stats", "gbsw_batch_autotune", "gbsw_align", "gbsw_device_memory",
+    "gbsw_version",
     "gbsw_server_config_default", "gbsw_server_create", "gbsw_server_destroy", "gbsw_server_submit",
     "gbsw_server_wait", "gbsw_server_poll", "gbsw_server_stats", "gbsw_host_alloc", "gbsw_host_free",
     "gbsw_context_set_option",
@@ -272,6 +273,14 @@ class Batch:
         out = np.zeros((5, max(self.n, 1)), dtype=np.int32)
         _check(self._lib.gbsw_batch_results(self._h, *[out[f].ctypes.data for f in range(5)]))
         return {f: out[k][:self.n].copy() for k, f in enumerate(FIELDS)}
+
+    def autotune(self, scoring, repeats=2):
+        """gbsw_batch_autotune: (chosen geometry model, its device ms)."""
+        m = ctypes.c_int32()
+        ms = ctypes.c_double()
+        _check(self._lib.gbsw_batch_autotune(self._h, ctypes.byref(scoring), int(repeats), ctypes.byref(m),
+                                             ctypes.byref(ms)))
+        return m.value, ms.value
 
     def stats(self):
         st = Stats()

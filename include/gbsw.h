@@ -87,9 +87,10 @@ int gbsw_context_destroy(gbsw_context* ctx);
 #define GBSW_OPT_F16 2
 /* GEOMETRY: how a pair's shorter length picks its kernel geometry (K columns per
  *          lane), for batches created afterwards.  0: fewest padded columns and
- *          steps, strips x (K + 2); 1 (default; GBSW_KMODEL=0 selects 0): that
- *          cost divided by the share of the VALU issue rate the geometry's
- *          occupancy allows.  Applies to the length buckets of
+ *          steps, strips x (K + 2); 1 (default; GBSW_KMODEL=m selects model m):
+ *          that cost divided by min(1, waves per SIMD / 2.5), the share of the
+ *          VALU issue rate the geometry's occupancy allows; 2..5: the same with
+ *          2.0, 3.0, 4.0 or 1.5 waves (gbsw_batch_autotune picks among them).  Applies to the length buckets of
  *          gbsw_batch_align(GBSW_BATCHED), and to the plain form for protein
  *          (the plain form uses 0 for DNA).  Results are identical. */
 #define GBSW_OPT_GEOMETRY 3
@@ -136,6 +137,15 @@ int gbsw_batch_results(gbsw_batch* batch, int32_t* score, int32_t* ref_begin, in
 int gbsw_batch_device_results(gbsw_batch* batch, const int32_t** d_out);
 
 int gbsw_batch_stats(const gbsw_batch* batch, gbsw_stats* out);
+
+/* Autotune the length -> geometry map on this batch: aligns it (BATCHED,
+ * `repeats` timed runs after one untimed) under each distinct map of the
+ * GBSW_OPT_GEOMETRY models, keeps the fastest (by device time; another model
+ * must beat the batch's current one by more than 1.5 %) for the batch and makes
+ * it the context's model for batches created afterwards.  *model and *ms
+ * (either may be null) receive the choice and its time.  Results do not depend
+ * on the map; the batch's result arrays hold those of the last run. */
+int gbsw_batch_autotune(gbsw_batch* batch, const gbsw_scoring* scoring, int32_t repeats, int32_t* model, double* ms);
 
 /* One kernel launch of the last gbsw_batch_align (index 0 .. stats.launches-1). */
 typedef struct {

@@ -336,6 +336,25 @@ def test_graph_replays_match_eager_results(ctx):
         m.close()
 
 
+def test_autotune_picks_a_model_and_keeps_results(ctx):
+    """gbsw_batch_autotune times the distinct length -> geometry maps on the batch
+    and keeps the fastest; the results stay the oracle's."""
+    w = synth.batched_dna(4000, seed=0x5EED020A)
+    exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, gbsw.DNA, w.scoring)
+    sc = gbsw.dna_scoring(*w.scoring)
+    b = ctx.batch(gbsw.DNA, w.ref, w.ref_off, w.query, w.query_off)
+    try:
+        print("case autotune", flush=True)
+        model, ms = b.autotune(sc, repeats=1)
+        assert 0 <= model < 6 and ms > 0
+        assert_same(b.results(), exp, "autotune runs")
+        b.align(sc, "batched")
+        assert_same(b.results(), exp, "after autotune")
+    finally:
+        b.close()
+        ctx.set_option(gbsw.OPT_GEOMETRY, 1)
+
+
 def test_geometry_models_give_identical_results(ctx):
     """Both length -> geometry models (the plain form of protein follows the
     option too): the same results."""
@@ -343,7 +362,7 @@ def test_geometry_models_give_identical_results(ctx):
         exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring)
         sc = gbsw_scoring(w.alphabet, w.scoring)
         try:
-            for model in (0, 1):
+            for model in range(6):
                 ctx.set_option(gbsw.OPT_GEOMETRY, model)
                 for mode in ("plain", "batched"):
                     print("case geometry", w.name, "model", model, mode, flush=True)
