@@ -122,7 +122,8 @@ int gbsw_context_set_option(gbsw_context* ctx, int32_t option, int64_t value)
     if (!ctx) return fail(GBSW_E_ARGUMENT, "context is null");
     switch (option) {
     case GBSW_OPT_STREAMS:
-        if (value < 1 || value > kAuxStreams + 1) return fail(GBSW_E_ARGUMENT, "streams must be 1..4");
+        if (value < 1 || value > kAuxStreams + 1) return fail(GBSW_E_ARGUMENT, "streams must be 1..8");
+        if (int rc = ctx->ex.ensure_streams((int)value)) return rc;
         ctx->ex.streams = (int)value;
         return GBSW_OK;
     case GBSW_OPT_F16:

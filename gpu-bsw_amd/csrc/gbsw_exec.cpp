@@ -94,7 +94,7 @@ int default_kmodel()
 int default_streams()
 {
     const char* e = std::getenv("GBSW_STREAMS");
-    const int v = e ? std::atoi(e) : kAuxStreams + 1;
+    const int v = e ? std::atoi(e) : kDefaultStreams;
     return std::max(1, std::min(kAuxStreams + 1, v));
 }
 
@@ -168,10 +168,7 @@ int Exec::init(int dev, int ncu)
     if (e == hipSuccess) e = hipEventCreate(&ev_start);
     if (e == hipSuccess) e = hipEventCreate(&ev_stop);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
-    for (int i = 0; i < kAuxStreams && e == hipSuccess; ++i) {
-        e = hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_join[i], hipEventDisableTiming);
-    }
+    if (e == hipSuccess && ensure_streams(streams) != GBSW_OK) e = hipErrorOutOfMemory;
     if (e == hipSuccess) e = status.reserve(sizeof(int32_t) * kStatusWords);
     if (e == hipSuccess) e = status_host.reserve(sizeof(int32_t) * kStatusWords);
     if (e == hipSuccess) e = tables_host.reserve(4096);
@@ -180,6 +177,16 @@ int Exec::init(int dev, int ncu)
     if (e == hipSuccess) e = lut.reserve(256);
     if (e == hipSuccess) e = colmax.reserve(sizeof(int16_t) * 32);
     if (e != hipSuccess) return fail(GBSW_E_DEVICE, std::string("stream/scratch setup: ") + hipGetErrorString(e));
+    return GBSW_OK;
+}
+
+int Exec::ensure_streams(int n)
+{
+    for (int i = 0; i < std::min(kAuxStreams, n - 1); ++i) {
+        if (aux[i]) continue;
+        GBSW_HIP_TRY(hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking));
+        GBSW_HIP_TRY(hipEventCreateWithFlags(&ev_join[i], hipEventDisableTiming));
+    }
     return GBSW_OK;
 }
 

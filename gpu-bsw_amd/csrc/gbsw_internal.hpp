@@ -61,15 +61,16 @@ struct LaunchRec {
     double ms;
 };
 
-constexpr int kAuxStreams = 3;
+constexpr int kAuxStreams = 7;   // at most 1 + 7 bucket streams
+constexpr int kDefaultStreams = 4;
 
 // Streams, events and scratch of one in-flight alignment job.
 struct Exec {
     int device = 0;
     int num_cu = 0;
     hipStream_t stream = nullptr;
-    hipStream_t aux[kAuxStreams] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev_fork = nullptr, ev_join[kAuxStreams] = {nullptr, nullptr, nullptr};
+    hipStream_t aux[kAuxStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kAuxStreams] = {};
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> kev; // per-launch event pairs
     DeviceBuffer bnd, queue, status, sub, lut, colmax, redo, split_buf;
@@ -78,7 +79,9 @@ struct Exec {
     std::vector<LaunchRec> launches;
     int32_t max_sub = 0;
     bool p8 = false;               // the current job uses the int8 profile
-    int streams = kAuxStreams + 1; // streams for the length buckets (1 = serialised)
+    int streams = kDefaultStreams; // streams for the length buckets (1 = serialised)
+    // creates the auxiliary streams the first n streams need
+    int ensure_streams(int n);
     bool f16 = true;               // exact f16 kernels where the score bound allows
     bool split = true;             // multi-strip buckets run their strips on several workgroups
     bool fixed = false;            // buffers preallocated (serving slot): never grown by a job

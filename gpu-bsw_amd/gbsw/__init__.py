@@ -224,7 +224,7 @@ class Context:
             pass
 
     def set_option(self, option, value):
-        """OPT_STREAMS (1..4), OPT_F16, OPT_GEOMETRY, OPT_SPLIT or OPT_GRAPHS (0/1); see include/gbsw.h."""
+        """OPT_STREAMS (1..8), OPT_F16, OPT_GEOMETRY (0..5), OPT_SPLIT or OPT_GRAPHS (0/1); see include/gbsw.h."""
         _check(self._lib.gbsw_context_set_option(self._h, option, int(value)))
 
     def batch(self, alphabet, ref, ref_off, query, query_off):

@@ -77,7 +77,7 @@ int gbsw_context_create(int32_t device, gbsw_context** out);
 int gbsw_context_destroy(gbsw_context* ctx);
 
 /* Context options (gbsw_context_set_option).
- * STREAMS: streams the length buckets of the batched form run on, 1..4 (default
+ * STREAMS: streams the length buckets of the batched form run on, 1..8 (default
  *          4, or the GBSW_STREAMS environment variable); 1 serialises them, so
  *          every launch is timed alone.
  * F16:     1 (default; GBSW_F16=0 turns it off): length buckets whose scores

@@ -281,7 +281,7 @@ def test_batched_options_give_identical_results(ctx):
         exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, w.alphabet, w.scoring)
         sc = gbsw_scoring(w.alphabet, w.scoring)
         try:
-            for streams, f16, split in ((4, 1, 1), (1, 1, 1), (4, 0, 1), (1, 0, 0), (4, 1, 0)):
+            for streams, f16, split in ((4, 1, 1), (1, 1, 1), (8, 0, 1), (1, 0, 0), (4, 1, 0), (8, 1, 1)):
                 print("case options", w.name, "streams", streams, "f16", f16, "split", split, flush=True)
                 ctx.set_option(gbsw.OPT_STREAMS, streams)
                 ctx.set_option(gbsw.OPT_F16, f16)
