@@ -76,10 +76,12 @@ __host__ __device__ inline int32_t align_up(int32_t x, int32_t a) { return (x + 
 
 // Profile layout of one row for K columns per virtual lane: K/8 blocks of 8
 // columns for all 32 virtual lanes (one b128 read each), then the K%8 remaining
-// columns for all virtual lanes padded to a b32/b64/b128 read (kremstride).
+// columns for all virtual lanes padded to a b32/b64/b96/b128 read (kremstride).
+// Six columns rather than eight for K%8 = 5, 6 keep the LDS of a DNA workgroup
+// at K = 5, 6 small enough for 12 workgroups per CU instead of 11.
 __host__ __device__ constexpr int32_t kremstride(int32_t K)
 {
-    return K % 8 == 0 ? 0 : K % 8 <= 2 ? 2 : K % 8 <= 4 ? 4 : 8;
+    return K % 8 == 0 ? 0 : K % 8 <= 2 ? 2 : K % 8 <= 4 ? 4 : K % 8 <= 6 ? 6 : 8;
 }
 // Profile columns reserved per virtual lane.
 __host__ __device__ constexpr int32_t kstride(int32_t K) { return K / 8 * 8 + kremstride(K); }

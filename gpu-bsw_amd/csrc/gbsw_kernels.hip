@@ -296,6 +296,9 @@ __device__ __forceinline__ void prof_load(const uint8_t* row, int v, uint32_t (&
     if constexpr (RS == 8) {
         const uint4 x = *(const uint4*)r;
         d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y; d[4 * Q + 2] = x.z; d[4 * Q + 3] = x.w;
+    } else if constexpr (RS == 6) {
+        const uint3 x = *(const uint3*)r;
+        d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y; d[4 * Q + 2] = x.z;
     } else if constexpr (RS == 4) {
         const uint2 x = *(const uint2*)r;
         d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y;
