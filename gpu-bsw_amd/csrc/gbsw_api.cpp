@@ -77,6 +77,28 @@ int build_alphabet(int32_t alphabet, const uint8_t* const* bytes, const uint64_t
     return std::max(A, 1);
 }
 
+// build_alphabet from the set of byte values present (bit b of mask[b / 32]),
+// as scanned on the device after the upload.
+int alphabet_from_mask(int32_t alphabet, const uint32_t* mask, uint8_t* lut)
+{
+    auto present = [&](int c) { return (mask[c >> 5] >> (c & 31)) & 1u; };
+    std::memset(lut, 0xFF, 256);
+    if (alphabet == GBSW_PROTEIN) {
+        for (int c = 91; c < 256; ++c)
+            if (present(c))
+                return fail(GBSW_E_RESIDUE, "protein residue byte >= 91 (the reference encoding table has 91 entries)");
+        uint8_t none[1] = {0};
+        const uint8_t* bufs[1] = {none};
+        const uint64_t lens[1] = {0};
+        return build_alphabet(alphabet, bufs, lens, 0, lut);
+    }
+    int A = 0;
+    for (int c = 0; c < 256; ++c)
+        if (present(c)) lut[c] = (uint8_t)A++;
+    if (A > kMaxAlpha) return fail(GBSW_E_RESIDUE, "more than 31 distinct DNA bytes in one batch");
+    return std::max(A, 1);
+}
+
 } // namespace gbsw
 
 extern "C" {
@@ -200,21 +222,8 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
     }
     const uint8_t* rp = (const uint8_t*)ref + (n ? ref_off[0] : 0);
     const uint8_t* qp = (const uint8_t*)query + (n ? query_off[0] : 0);
-    {
-        const uint8_t* bufs[2] = {rp, qp};
-        const uint64_t lens[2] = {ref_bytes, qry_bytes};
-        const int A = build_alphabet(alphabet, bufs, lens, 2, b->lut);
-        if (A < 0) {
-            const std::string err = gbsw_last_error();
-            delete b;
-            set_error(err);
-            return A;
-        }
-        job.alpha = A;
-    }
     job.lut_host = b->lut;
     job.kmodel = ctx->ex.kmodel;
-    count_buckets(job, b->len_ref.data(), b->len_qry.data());
     size_t sort_bytes = 0;
     int rc = sort_scratch_bytes(n, &sort_bytes);
     if (rc) {
@@ -227,7 +236,7 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
     if (e == hipSuccess) e = b->ref_off.reserve(sizeof(uint64_t) * (np + 1));
     if (e == hipSuccess) e = b->qry_off.reserve(sizeof(uint64_t) * (np + 1));
     if (e == hipSuccess) e = b->out.reserve(sizeof(int32_t) * 5 * np);
-    if (e == hipSuccess) e = b->keys.reserve(4 * np);
+    if (e == hipSuccess) e = b->keys.reserve(std::max<size_t>(4 * np, 32));
     if (e == hipSuccess) e = b->keys_sorted.reserve(4 * np);
     if (e == hipSuccess) e = b->vals.reserve(4 * np);
     if (e == hipSuccess) e = b->order.reserve(4 * np);
@@ -238,12 +247,39 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
     if (e == hipSuccess) e = hipMemcpy(b->ref_off.ptr, ro.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b->qry_off.ptr, qo.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(b->out.ptr, 0, sizeof(int32_t) * 5 * np);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, job.kmodel).data(), 65536, hipMemcpyHostToDevice);
+    // The residue codes need the set of byte values in the batch (every distinct
+    // DNA byte is a code; protein bytes must be < 91): a scan of the uploaded
+    // bytes on the device instead of a pass over them on the host.
+    uint32_t seen[8] = {0};
+    if (e == hipSuccess) {
+        hipStream_t st = ctx->ex.stream;
+        uint32_t* dmask = b->keys.as<uint32_t>();
+        e = zero_u32(dmask, 8, st);
+        if (e == hipSuccess) e = byte_presence(b->ref.as<uint8_t>(), ref_bytes, dmask, st);
+        if (e == hipSuccess) e = byte_presence(b->qry.as<uint8_t>(), qry_bytes, dmask, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(seen, dmask, sizeof(seen), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
     if (e != hipSuccess) {
         delete b;
         return fail(e == hipErrorOutOfMemory ? GBSW_E_MEMORY : GBSW_E_DEVICE,
                     std::string("batch upload: ") + hipGetErrorString(e));
+    }
+    {
+        const int A = alphabet_from_mask(alphabet, seen, b->lut);
+        if (A < 0) {
+            const std::string err = gbsw_last_error();
+            delete b;
+            set_error(err);
+            return A;
+        }
+        job.alpha = A;
+    }
+    count_buckets(job, b->len_ref.data(), b->len_qry.data());
+    e = hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, job.kmodel).data(), 65536, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(GBSW_E_DEVICE, std::string("batch upload: ") + hipGetErrorString(e));
     }
     job.ref = b->ref.as<uint8_t>();
     job.qry = b->qry.as<uint8_t>();

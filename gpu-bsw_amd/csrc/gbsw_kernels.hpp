@@ -37,6 +37,8 @@ struct StripWidths {
 };
 hipError_t bin_keys(const uint64_t* ref_off, const uint64_t* qry_off, int64_t n, const uint8_t* bucket_of_len,
                     const StripWidths& widths, uint32_t* keys, int32_t* vals, hipStream_t stream);
+// OR into mask[8] the set of byte values in p[0, n) (p 16-byte aligned).
+hipError_t byte_presence(const uint8_t* p, uint64_t n, uint32_t* mask, hipStream_t stream);
 hipError_t sort_pairs(void* temp, size_t& temp_bytes, const uint32_t* kin, uint32_t* kout, const int32_t* vin,
                       int32_t* vout, int n, int end_bit, hipStream_t stream);
 

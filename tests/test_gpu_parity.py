@@ -208,6 +208,14 @@ def test_errors_are_reported(ctx):
     with pytest.raises(gbsw.GbswError) as e:
         ctx.align(["ACGT"], ["ACGT"], gbsw.dna_scoring(1, -1, 2, -1))  # positive gap score
     assert e.value.code == -3
+    # 32 distinct DNA bytes (found by the device scan of the uploaded batch)
+    many = "".join(chr(c) for c in range(65, 97))
+    with pytest.raises(gbsw.GbswError) as e:
+        ctx.align([many], ["ACGT"], gbsw.dna_scoring(3, -3, -6, -1))
+    assert e.value.code == -2
+    # 31 distinct bytes are fine, in their byte order
+    got = ctx.align([many[:31]], [many[5:20]], gbsw.dna_scoring(3, -3, -6, -1))
+    assert int(got["score"][0]) == 45 and int(got["ref_begin"][0]) == 5
 
 
 def test_drop_in_python_driver(ctx):
