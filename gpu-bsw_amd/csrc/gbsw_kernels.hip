@@ -603,17 +603,22 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             {
                 const int ck = s0 / kChunk + 1;
                 row_store<A>(pf, ck, l, ring_code, ring_bnd, lut, alpha, rowstride);
-                if constexpr (SPLIT) {
-                    // the rows flushed one chunk ago have long been stored: publish
-                    if (s0 > kChunk && strip + 1 < strips_max)
-                        publish_word(io.prog_out, io.epoch_out | (uint64_t)(uint32_t)(s0 - 2 * kChunk + 1));
-                }
                 need_rows((ck + 2) * kChunk);
                 pf = row_fetch<A>(t, ck + 1, l, bnd_rd, use_bnd);
             }
             __syncthreads();
             // rows leaving the last virtual lane during the previous chunk
-            if (multi_strip && s0 > 0) flush_bnd(s0 - kChunk - (kVirtPerPair - 1));
+            if (multi_strip && s0 > 0) {
+                flush_bnd(s0 - kChunk - (kVirtPerPair - 1));
+                if constexpr (SPLIT) {
+                    // rows below s0 - 31 are stored once the flush completes: the
+                    // wait for it (publish_word) is hidden by the other waves of
+                    // the SIMD, and the next strip starts a chunk earlier than if
+                    // the rows were published one chunk later
+                    if (strip + 1 < strips_max)
+                        publish_word(io.prog_out, io.epoch_out | (uint64_t)(uint32_t)(s0 - (kVirtPerPair - 1)));
+                }
+            }
             // two steps per iteration (manual unroll: the DPP moves are convergent,
             // so the compiler will not unroll a runtime trip count); a step past
             // steps_max only meets padding rows
