@@ -13,9 +13,14 @@ constexpr int kChunk = 32;   // rows refilled at a time
 // The row-code ring keeps a copy of its first kRingTail entries after its end,
 // so the entries a virtual lane reads during one chunk (32 steps, one step more
 // when the step count is odd, and the look-ahead of one step) are contiguous
-// whatever the chunk's position in the ring.
-constexpr int kRingTail = kChunk + 2;
+// whatever the chunk's position in the ring, also when read 16 entries on from
+// the position of the virtual lane 16 higher (both halves of a lane use one
+// pointer).
+constexpr int kRingTail = kChunk + kLanesPerPair + 2;
 constexpr int kRingCodes = kRing + kRingTail;
+// Boundary-column ring: the look-ahead read of the step after a chunk's last
+// may pass the ring's end by one or two entries (never used).
+constexpr int kRingBnd = kRing + 2;
 constexpr int kMaxAlpha = 31; // codes 0..A-1, row A is the padding row
 constexpr int16_t kNeg = -16384;
 
@@ -119,7 +124,7 @@ __host__ __device__ inline LdsLayout lds_layout(int32_t row_bytes, int32_t alpha
     L.prof_bytes = align_up((alpha + 1) * row_bytes, 128);
     L.ring_code_off = L.prof_bytes;
     L.ring_bnd_off = L.ring_code_off + kRingCodes * 2;
-    L.group_bytes = align_up(L.ring_bnd_off + kRing * bnd_bytes, 128);
+    L.group_bytes = align_up(L.ring_bnd_off + kRingBnd * bnd_bytes, 128);
     L.total = L.group_off + kPairsPerWave * L.group_bytes;
     return L;
 }

@@ -30,6 +30,8 @@
 //    chosen after the pass.
 //  reverse: the same DP on the reversed prefixes; among cells with H = score, the
 //    smallest reversed column, then its first row (kernel.cpp:3-45, 577-581).
+#include <type_traits>
+
 #include "gbsw_device.hpp"
 #include "gbsw_kernel_args.hpp"
 #include "gbsw_kernels.hpp"
@@ -598,135 +600,151 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
             }
         };
 
-        for (int s0 = 0; s0 < steps_max; s0 += kChunk) {
-            // refill: rows of chunk (s0/32 + 1) enter the ring, chunk +2 is requested
-            {
-                const int ck = s0 / kChunk + 1;
-                row_store<A>(pf, ck, l, ring_code, ring_bnd, lut, alpha, rowstride);
-                need_rows((ck + 2) * kChunk);
-                pf = row_fetch<A>(t, ck + 1, l, bnd_rd, use_bnd);
-            }
-            __syncthreads();
-            // rows leaving the last virtual lane during the previous chunk
-            if (multi_strip && s0 > 0) {
-                flush_bnd(s0 - kChunk - (kVirtPerPair - 1));
-                if constexpr (SPLIT) {
-                    // rows below s0 - 31 are stored once the flush completes: the
-                    // wait for it (publish_word) is hidden by the other waves of
-                    // the SIMD, and the next strip starts a chunk earlier than if
-                    // the rows were published one chunk later
-                    if (strip + 1 < strips_max)
-                        publish_word(io.prog_out, io.epoch_out | (uint64_t)(uint32_t)(s0 - (kVirtPerPair - 1)));
+        // waves of single-strip pairs read no boundary column and hand none on; the
+        // two forms are separate loops so that neither holds the other's registers
+        auto chunks = [&](auto ms) {
+            for (int s0 = 0; s0 < steps_max; s0 += kChunk) {
+                // refill: rows of chunk (s0/32 + 1) enter the ring, chunk +2 is requested
+                {
+                    const int ck = s0 / kChunk + 1;
+                    row_store<A>(pf, ck, l, ring_code, ring_bnd, lut, alpha, rowstride);
+                    need_rows((ck + 2) * kChunk);
+                    pf = row_fetch<A>(t, ck + 1, l, bnd_rd, use_bnd);
                 }
-            }
-            // two steps per iteration (manual unroll: the DPP moves are convergent,
-            // so the compiler will not unroll a runtime trip count); a step past
-            // steps_max only meets padding rows
-            const int s_end = min(s0 + kChunk, steps_max);
-            // profile words and strip boundary of one step, requested a step ahead
-            struct StepIn {
-                uint32_t plo[NW], phi[NW];
-                Bd bnd;
-            };
-            auto fetch = [&](StepIn& in, const int s, const uint32_t off_lo, const uint32_t off_hi) {
-                if constexpr (P8) {
-                    prof_load8<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
-                    prof_load8<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
-                } else {
-                    prof_load<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
-                    prof_load<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
-                }
-                in.bnd = ring_bnd[s & (kRing - 1)]; // zero unless a strip precedes
-            };
-            auto step = [&](const int s, const StepIn& in) {
-                const uint32_t* plo = in.plo;
-                const uint32_t* phi = in.phi;
-                const Bd bnd = in.bnd;
-                // left inputs: neighbour's outputs of the previous step; virtual lane 0
-                // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
-                V xH = A::dpp(Hout), xE = A::dpp(Eout);
-                xH = A::fix_h(xH, bnd, lfix);
-                xE = A::fix_e(xE, bnd, lfix);
-
-                V E = xE, Hd = Hdiag0;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    V S;
-                    if constexpr (P8) S = A::template prof8<K>(plo, phi, k);
-                    else S = A::template prof<K>(plo, phi, k);
-                    const V D = A::add(Hd, S);
-                    Hd = Hcol[k];
-                    const V H = A::cell(D, E, Fcol[k]);
-                    const V Ho = A::open_in(H, OPEN2);
-                    E = A::decay(E, EXT2, Ho);
-                    Fcol[k] = A::decay(Fcol[k], EXT2, Ho);
-                    Hcol[k] = H;
-                }
-                Hout = Hcol[K - 1];
-                Eout = E;
-                Hdiag0 = xH;
-
-                // the last virtual lane hands its column to the next strip through
-                // its row's ring slot (whose incoming entry was read 31 steps ago);
-                // flush_bnd writes them out a chunk at a time
-                if (multi_strip && write_bnd && l == 15) ring_bnd[(s - v_hi) & (kRing - 1)] = A::bnd_out(Hout, Eout);
-
-                const V rm = tree_max<A, K>(Hcol);
-                if constexpr (!REV) {
-                    const V m = A::ltmask(best, rm);
-                    best = A::vmax(best, rm);
-                    bstep = A::sel_step(m, s, bstep);
-#pragma unroll
-                    for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
-                } else {
-                    if (A::eq_lo(rm, TGT) || A::eq_hi(rm, TGT)) {
-                        const int row_lo = s - v_lo, row_hi = s - v_hi;
-                        // first (smallest) column of this row that reaches the target
-                        int k_lo = K, k_hi = K;
-                        if constexpr (!A::kWide && K <= 16) {
-                            // bit k of each half: column k equals the target
-                            uint32_t acc = 0;
-#pragma unroll
-                            for (int k = K - 1; k >= 0; --k)
-                                acc = raw(as_u(acc) * (u16x2){2, 2} + as_u(A::eq_bits(Hcol[k], TGT)));
-                            const uint32_t ml = (acc & 0xFFFFu) & cols_mask(t.nc - (strip_base + v_lo * K));
-                            const uint32_t mh = (acc >> 16) & cols_mask(t.nc - (strip_base + v_hi * K));
-                            k_lo = ml ? __builtin_ctz(ml) : K;
-                            k_hi = mh ? __builtin_ctz(mh) : K;
-                        } else {
-#pragma unroll
-                            for (int k = K - 1; k >= 0; --k) {
-                                if (A::eq_lo(Hcol[k], TGT) && strip_base + v_lo * K + k < t.nc) k_lo = k;
-                                if (A::eq_hi(Hcol[k], TGT) && strip_base + v_hi * K + k < t.nc) k_hi = k;
-                            }
-                        }
-                        if (k_lo < K && row_lo >= 0 && row_lo < t.nr && strip_base + v_lo * K + k_lo < rt_lo) {
-                            rt_lo = strip_base + v_lo * K + k_lo;
-                            rr_lo = row_lo;
-                        }
-                        if (k_hi < K && row_hi >= 0 && row_hi < t.nr && strip_base + v_hi * K + k_hi < rt_hi) {
-                            rt_hi = strip_base + v_hi * K + k_hi;
-                            rr_hi = row_hi;
-                        }
+                __syncthreads();
+                // rows leaving the last virtual lane during the previous chunk
+                if (multi_strip && s0 > 0) {
+                    flush_bnd(s0 - kChunk - (kVirtPerPair - 1));
+                    if constexpr (SPLIT) {
+                        // rows below s0 - 31 are stored once the flush completes: the
+                        // wait for it (publish_word) is hidden by the other waves of
+                        // the SIMD, and the next strip starts a chunk earlier than if
+                        // the rows were published one chunk later
+                        if (strip + 1 < strips_max)
+                            publish_word(io.prog_out, io.epoch_out | (uint64_t)(uint32_t)(s0 - (kVirtPerPair - 1)));
                     }
                 }
-            };
-            // this chunk's row codes of each virtual lane are contiguous in the ring
-            // (software pipeline: the LDS reads of step s + 1 are in flight while
-            // step s computes)
-            const uint16_t* rc_lo = ring_code + ((s0 - v_lo) & (kRing - 1));
-            const uint16_t* rc_hi = ring_code + ((s0 - v_hi) & (kRing - 1));
-            StepIn ia, ib;
-            fetch(ia, s0, rc_lo[0], rc_hi[0]);
-            for (int s = s0; s < s_end; s += 2) {
-                fetch(ib, s + 1, rc_lo[1], rc_hi[1]);
-                step(s, ia);
-                fetch(ia, s + 2, rc_lo[2], rc_hi[2]);
-                step(s + 1, ib);
-                rc_lo += 2;
-                rc_hi += 2;
+                // two steps per iteration (manual unroll: the DPP moves are convergent,
+                // so the compiler will not unroll a runtime trip count); a step past
+                // steps_max only meets padding rows
+                const int s_end = min(s0 + kChunk, steps_max);
+                // profile words and strip boundary of one step, requested a step ahead
+                struct StepIn {
+                    uint32_t plo[NW], phi[NW];
+                    Bd bnd;
+                };
+                auto fetch = [&](StepIn& in, const uint32_t off_lo, const uint32_t off_hi, const Bd* pb, auto ms) {
+                    if constexpr (P8) {
+                        prof_load8<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
+                        prof_load8<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
+                    } else {
+                        prof_load<K>((const uint8_t*)prof + off_lo, v_lo, in.plo);
+                        prof_load<K>((const uint8_t*)prof + off_hi, v_hi, in.phi);
+                    }
+                    // the strip's left boundary column (zero in the first strip)
+                    if constexpr (decltype(ms)::value) in.bnd = *pb;
+                    else in.bnd = A::zero_bnd();
+                };
+                auto step = [&](const int s, const StepIn& in, auto ms) {
+                    const uint32_t* plo = in.plo;
+                    const uint32_t* phi = in.phi;
+                    const Bd bnd = in.bnd;
+                    // left inputs: neighbour's outputs of the previous step; virtual lane 0
+                    // takes the strip's left boundary, virtual lane 16 takes lane 15's low half
+                    V xH = A::dpp(Hout), xE = A::dpp(Eout);
+                    xH = A::fix_h(xH, bnd, lfix);
+                    xE = A::fix_e(xE, bnd, lfix);
+
+                    V E = xE, Hd = Hdiag0;
+    #pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        V S;
+                        if constexpr (P8) S = A::template prof8<K>(plo, phi, k);
+                        else S = A::template prof<K>(plo, phi, k);
+                        const V D = A::add(Hd, S);
+                        Hd = Hcol[k];
+                        const V H = A::cell(D, E, Fcol[k]);
+                        const V Ho = A::open_in(H, OPEN2);
+                        E = A::decay(E, EXT2, Ho);
+                        Fcol[k] = A::decay(Fcol[k], EXT2, Ho);
+                        Hcol[k] = H;
+                    }
+                    Hout = Hcol[K - 1];
+                    Eout = E;
+                    Hdiag0 = xH;
+
+                    // the last virtual lane hands its column to the next strip through
+                    // its row's ring slot (whose incoming entry was read 31 steps ago);
+                    // flush_bnd writes them out a chunk at a time
+                    if constexpr (decltype(ms)::value) {
+                        if (write_bnd && l == 15) ring_bnd[(s - v_hi) & (kRing - 1)] = A::bnd_out(Hout, Eout);
+                    }
+
+                    const V rm = tree_max<A, K>(Hcol);
+                    if constexpr (!REV) {
+                        const V m = A::ltmask(best, rm);
+                        best = A::vmax(best, rm);
+                        bstep = A::sel_step(m, s, bstep);
+    #pragma unroll
+                        for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
+                    } else {
+                        if (A::eq_lo(rm, TGT) || A::eq_hi(rm, TGT)) {
+                            const int row_lo = s - v_lo, row_hi = s - v_hi;
+                            // first (smallest) column of this row that reaches the target
+                            int k_lo = K, k_hi = K;
+                            if constexpr (!A::kWide && K <= 16) {
+                                // bit k of each half: column k equals the target
+                                uint32_t acc = 0;
+    #pragma unroll
+                                for (int k = K - 1; k >= 0; --k)
+                                    acc = raw(as_u(acc) * (u16x2){2, 2} + as_u(A::eq_bits(Hcol[k], TGT)));
+                                const uint32_t ml = (acc & 0xFFFFu) & cols_mask(t.nc - (strip_base + v_lo * K));
+                                const uint32_t mh = (acc >> 16) & cols_mask(t.nc - (strip_base + v_hi * K));
+                                k_lo = ml ? __builtin_ctz(ml) : K;
+                                k_hi = mh ? __builtin_ctz(mh) : K;
+                            } else {
+    #pragma unroll
+                                for (int k = K - 1; k >= 0; --k) {
+                                    if (A::eq_lo(Hcol[k], TGT) && strip_base + v_lo * K + k < t.nc) k_lo = k;
+                                    if (A::eq_hi(Hcol[k], TGT) && strip_base + v_hi * K + k < t.nc) k_hi = k;
+                                }
+                            }
+                            if (k_lo < K && row_lo >= 0 && row_lo < t.nr && strip_base + v_lo * K + k_lo < rt_lo) {
+                                rt_lo = strip_base + v_lo * K + k_lo;
+                                rr_lo = row_lo;
+                            }
+                            if (k_hi < K && row_hi >= 0 && row_hi < t.nr && strip_base + v_hi * K + k_hi < rt_hi) {
+                                rt_hi = strip_base + v_hi * K + k_hi;
+                                rr_hi = row_hi;
+                            }
+                        }
+                    }
+                };
+                // this chunk's row codes are contiguous in the ring, those of virtual
+                // lane v_lo 16 entries on from those of v_hi, and so are its boundary
+                // entries (software pipeline: the LDS reads of step s + 1 are in flight
+                // while step s computes)
+                const uint16_t* rc0 = ring_code + ((s0 - v_hi) & (kRing - 1));
+                const Bd* pb0 = ring_bnd + (s0 & (kRing - 1));
+                auto steps = [&](auto ms) {
+                    const uint16_t* rc = rc0;
+                    const Bd* pb = pb0;
+                    StepIn ia, ib;
+                    fetch(ia, rc[16], rc[0], pb, ms);
+                    for (int s = s0; s < s_end; s += 2) {
+                        fetch(ib, rc[17], rc[1], pb + 1, ms);
+                        step(s, ia, ms);
+                        fetch(ia, rc[18], rc[2], pb + 2, ms);
+                        step(s + 1, ib, ms);
+                        rc += 2;
+                        pb += 2;
+                    }
+                };
+                steps(ms);
             }
-        }
+        };
+        if (multi_strip) chunks(std::true_type{});
+        else chunks(std::false_type{});
         if (multi_strip) {
             // the last chunk's rows; complete before the next strip reads them
             __syncthreads();
@@ -948,7 +966,12 @@ __device__ __forceinline__ int64_t slot_pair(const KernelArgs& a, int64_t slot)
 template <class A, int K, bool P8>
 __device__ __forceinline__ Smem make_smem(uint8_t* raw, const KernelArgs& a)
 {
-    return Smem{raw, lds_layout(prof_row_bytes(K, P8), a.alpha, (int)sizeof(typename A::B))};
+    LdsLayout L = lds_layout(prof_row_bytes(K, P8), a.alpha, (int)sizeof(typename A::B));
+    // An opaque group offset: the compiler would otherwise split the rounding
+    // constants of the layout into the profile address, leaving an odd byte offset
+    // that ds_read2 cannot encode and one extra add per profile read.
+    asm volatile("" : "+s"(L.group_off));
+    return Smem{raw, L};
 }
 
 template <class A>
@@ -1205,16 +1228,25 @@ static const void* f16_ptr()
     else return (const void*)&k_batched<PackedF16, K, false>;
 }
 
+// Compiled geometries (a build may narrow the lists with -D for inspection).
+#ifndef GBSW_K_LIST
 #define GBSW_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
+#endif
 
+#ifndef GBSW_WIDE_K_LIST
 #define GBSW_WIDE_K_LIST(X) X(4) X(8) X(16)
+#endif
 
 // f16 kernels: the geometries whose single strip (32 x K columns) keeps DNA-like
 // scores below 2048
+#ifndef GBSW_F16_K_LIST
 #define GBSW_F16_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20)
+#endif
 
 // split kernels: the geometries multi-strip buckets use
+#ifndef GBSW_SPLIT_K_LIST
 #define GBSW_SPLIT_K_LIST(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16)
+#endif
 
 template <bool P8>
 static const void* kernel_for_profile(int kind, int K)
