@@ -314,3 +314,188 @@ int64_t gbsw_oracle_align_batch(int64_t n, const char* ref, const uint64_t* ref_
     }
     return first_bad ? -first_bad : 0;
 }
+
+/* ------------------------------------------------------------------ paths
+ *
+ * Alignment path of one pair: a capability the reference declares but never
+ * implements (traceBack, kernel.hpp:36-39, no definition in kernel.cpp).  This
+ * build defines it as the highest-scoring GLOBAL alignment of
+ * ref[ref_begin, ref_end) against query[query_begin, query_end) (the rectangle
+ * the forward and reverse passes report), scored like the forward pass: a run
+ * of L gap positions scores gap_open + (L-1) gap_extend, and the substitution is
+ * s(row byte, column byte) with rows = the longer full sequence (ref on ties,
+ * kernel.cpp:248-259; the protein table is indexed enc(row)*24 + enc(col),
+ * kernel.cpp:785-788).  Among optimal alignments the path is the one whose op
+ * sequence, read from its end backwards, is lexicographically smallest with
+ * M < D < I (M = a residue pair, D = a ref residue against a gap, I = a query
+ * residue against a gap).
+ *
+ * This restatement keeps the full H/E/F matrices and decides each backward step
+ * from their values (the device engine stores tightness bits instead). */
+
+#define TB_NEG (-(1 << 29))
+
+typedef struct {
+    int32_t *H, *E, *F;
+    uint8_t* ops;
+    size_t cap_cells, cap_ops;
+} tb_workspace;
+
+static int tb_reserve(tb_workspace* w, size_t cells, size_t nops)
+{
+    if (cells > w->cap_cells) {
+        free(w->H); free(w->E); free(w->F);
+        w->cap_cells = cells;
+        w->H = (int32_t*)malloc(sizeof(int32_t) * cells);
+        w->E = (int32_t*)malloc(sizeof(int32_t) * cells);
+        w->F = (int32_t*)malloc(sizeof(int32_t) * cells);
+        if (!w->H || !w->E || !w->F) return -1;
+    }
+    if (nops > w->cap_ops) {
+        free(w->ops);
+        w->cap_ops = nops;
+        w->ops = (uint8_t*)malloc(nops);
+        if (!w->ops) return -1;
+    }
+    return 0;
+}
+
+static void tb_free(tb_workspace* w)
+{
+    free(w->H); free(w->E); free(w->F); free(w->ops);
+    memset(w, 0, sizeof(*w));
+}
+
+static int32_t tb_sub(const gbsw_oracle_scoring* sc, char rowc, char colc)
+{
+    if (sc->kind == GBSW_ORACLE_AA)
+        return sc->matrix[gbsw_oracle_encode_aa((signed char)rowc) * 24 + gbsw_oracle_encode_aa((signed char)colc)];
+    return rowc == colc ? sc->match : sc->mismatch;
+}
+
+enum { TB_M = 0, TB_I = 1, TB_D = 2 }; /* BAM CIGAR op codes */
+
+static int tb_path(const char* a, int32_t m, const char* b, int32_t n, int swap, const gbsw_oracle_scoring* sc,
+                   tb_workspace* w, uint32_t* ops, int32_t cap, int32_t* nops, int32_t* path_score)
+{
+    const int32_t open = sc->gap_open, ext = sc->gap_extend;
+    const size_t W = (size_t)n + 1;
+    if (sc->kind == GBSW_ORACLE_AA) {
+        for (int32_t i = 0; i < m; ++i) if (gbsw_oracle_encode_aa((signed char)a[i]) < 0) return -1;
+        for (int32_t j = 0; j < n; ++j) if (gbsw_oracle_encode_aa((signed char)b[j]) < 0) return -1;
+    }
+    if (tb_reserve(w, ((size_t)m + 1) * W, (size_t)m + (size_t)n + 1)) return -2;
+    int32_t *H = w->H, *E = w->E, *F = w->F;
+#define S_(i, j) (swap ? tb_sub(sc, b[(j) - 1], a[(i) - 1]) : tb_sub(sc, a[(i) - 1], b[(j) - 1]))
+    H[0] = 0; E[0] = F[0] = TB_NEG;
+    for (int32_t j = 1; j <= n; ++j) { E[j] = open + (j - 1) * ext; H[j] = E[j]; F[j] = TB_NEG; }
+    for (int32_t i = 1; i <= m; ++i) {
+        const size_t r = (size_t)i * W, u = r - W;
+        F[r] = open + (i - 1) * ext; H[r] = F[r]; E[r] = TB_NEG;
+        for (int32_t j = 1; j <= n; ++j) {
+            const int32_t e = imax(E[r + j - 1] + ext, H[r + j - 1] + open);
+            const int32_t f = imax(F[u + j] + ext, H[u + j] + open);
+            const int32_t d = H[u + j - 1] + S_(i, j);
+            E[r + j] = e; F[r + j] = f;
+            H[r + j] = imax(d, imax(e, f));
+        }
+    }
+    *path_score = H[(size_t)m * W + n];
+    /* backward walk: state 0 = H, 1 = F (in a D run), 2 = E (in an I run) */
+    int32_t i = m, j = n, k = 0, st = 0;
+#define AT(X, i, j) X[(size_t)(i) * W + (j)]
+    while (i > 0 || j > 0) {
+        if (i == 0) { w->ops[k++] = TB_I; --j; continue; }
+        if (j == 0) { w->ops[k++] = TB_D; --i; continue; }
+        if (st == 0) {
+            if (AT(H, i, j) == AT(H, i - 1, j - 1) + S_(i, j)) { w->ops[k++] = TB_M; --i; --j; }
+            else if (AT(H, i, j) == AT(F, i, j)) st = 1;
+            else st = 2;
+            continue;
+        }
+        if (st == 1) {
+            w->ops[k++] = TB_D;
+            const int open_t = AT(F, i, j) == AT(H, i - 1, j) + open;
+            const int ext_t = AT(F, i, j) == AT(F, i - 1, j) + ext;
+            const int m_next = open_t && i >= 2 && AT(H, i - 1, j) == AT(H, i - 2, j - 1) + S_(i - 1, j);
+            st = m_next ? 0 : ext_t ? 1 : 0;
+            --i;
+            continue;
+        }
+        w->ops[k++] = TB_I;
+        {
+            const int open_t = AT(E, i, j) == AT(H, i, j - 1) + open;
+            const int ext_t = AT(E, i, j) == AT(E, i, j - 1) + ext;
+            const int md_next = open_t && j >= 2 &&
+                                (AT(H, i, j - 1) == AT(H, i - 1, j - 2) + S_(i, j - 1) || AT(H, i, j - 1) == AT(F, i, j - 1));
+            st = md_next ? 0 : ext_t ? 2 : 0;
+            --j;
+        }
+    }
+#undef AT
+#undef S_
+    /* forward order, run-length encoded as BAM ops (length << 4 | op) */
+    int32_t c = 0;
+    for (int32_t x = k - 1; x >= 0;) {
+        int32_t y = x;
+        while (y > 0 && w->ops[y - 1] == w->ops[x]) --y;
+        if (c >= cap) return -3;
+        ops[c++] = ((uint32_t)(x - y + 1) << 4) | w->ops[x];
+        x = y - 1;
+    }
+    *nops = c;
+    return 0;
+}
+
+int gbsw_oracle_traceback(const char* ref, int32_t ref_len, const char* query, int32_t query_len,
+                          const gbsw_oracle_scoring* sc, const gbsw_oracle_result* res, uint32_t* ops,
+                          int32_t cap, int32_t* nops, int32_t* path_score)
+{
+    tb_workspace w; memset(&w, 0, sizeof(w));
+    *nops = 0; *path_score = 0;
+    int rc = 0;
+    if (res->score > 0)
+        rc = tb_path(ref + res->ref_begin, res->ref_end - res->ref_begin, query + res->query_begin,
+                     res->query_end - res->query_begin, ref_len < query_len, sc, &w, ops, cap, nops, path_score);
+    tb_free(&w);
+    return rc;
+}
+
+int64_t gbsw_oracle_traceback_batch(int64_t n, const char* ref, const uint64_t* ref_off, const char* query,
+                                    const uint64_t* query_off, const gbsw_oracle_scoring* sc,
+                                    const gbsw_oracle_result* res, uint32_t* ops, const int64_t* cap_off,
+                                    int32_t* nops, int32_t* path_score, int nthreads)
+{
+    int64_t first_bad = 0;
+    const int nt = gbsw_oracle_threads(nthreads);
+    (void)nt;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt)
+#endif
+    {
+        tb_workspace w; memset(&w, 0, sizeof(w));
+        int64_t bad = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int64_t k = 0; k < n; ++k) {
+            const gbsw_oracle_result* r = &res[k];
+            const int32_t lr = (int32_t)(ref_off[k + 1] - ref_off[k]), lq = (int32_t)(query_off[k + 1] - query_off[k]);
+            nops[k] = 0; path_score[k] = 0;
+            int rc = 0;
+            if (r->score > 0)
+                rc = tb_path(ref + ref_off[k] + r->ref_begin, r->ref_end - r->ref_begin,
+                             query + query_off[k] + r->query_begin, r->query_end - r->query_begin, lr < lq, sc, &w,
+                             ops + cap_off[k], (int32_t)(cap_off[k + 1] - cap_off[k]), &nops[k], &path_score[k]);
+            if (rc && (bad == 0 || k + 1 < bad)) bad = k + 1;
+        }
+        tb_free(&w);
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            if (bad && (first_bad == 0 || bad < first_bad)) first_bad = bad;
+        }
+    }
+    return first_bad ? -first_bad : 0;
+}

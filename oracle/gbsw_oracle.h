@@ -72,6 +72,26 @@ int64_t gbsw_oracle_align_batch(int64_t n, const char* ref, const uint64_t* ref_
                                 const gbsw_oracle_scoring* sc, gbsw_oracle_result* out,
                                 int nthreads);
 
+/* Alignment path of one aligned pair (no counterpart in the reference, which
+ * only declares traceBack, kernel.hpp:36-39): the best global alignment of
+ * ref[res->ref_begin, res->ref_end) against query[res->query_begin,
+ * res->query_end) under the forward pass's scoring and orientation; among
+ * optimal ones, the op sequence read from its end is lexicographically smallest
+ * with M < D < I.  ops receive BAM CIGAR words (length << 4 | op; op 0 = M,
+ * 1 = I, 2 = D), at most cap; *path_score its score.  Pairs with score 0 give no
+ * ops.  Returns 0, -1 on an invalid residue, -3 when cap is too small. */
+int gbsw_oracle_traceback(const char* ref, int32_t ref_len, const char* query, int32_t query_len,
+                          const gbsw_oracle_scoring* sc, const gbsw_oracle_result* res, uint32_t* ops,
+                          int32_t cap, int32_t* nops, int32_t* path_score);
+
+/* Batch form: pair k writes at most cap_off[k+1] - cap_off[k] words at
+ * ops + cap_off[k], their count to nops[k].  Returns 0 or -(k+1) for the first
+ * failing pair. */
+int64_t gbsw_oracle_traceback_batch(int64_t n, const char* ref, const uint64_t* ref_off, const char* query,
+                                    const uint64_t* query_off, const gbsw_oracle_scoring* sc,
+                                    const gbsw_oracle_result* res, uint32_t* ops, const int64_t* cap_off,
+                                    int32_t* nops, int32_t* path_score, int nthreads);
+
 /* Threads an OpenMP region actually gets for nthreads (<= 0: all). */
 int gbsw_oracle_threads(int nthreads);
 
