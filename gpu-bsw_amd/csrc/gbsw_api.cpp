@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -46,6 +47,9 @@ struct gbsw_batch {
     std::vector<LaunchRec> launches; // of the last gbsw_batch_align
     JobGraph graph;
     bool one_shot = false; // a gbsw_align batch: aligned once, never worth a graph
+    bool aligned = false;  // results of a gbsw_batch_align are in `out`
+    // alignment paths: per-pair tables, run buffers, tightness scratch
+    DeviceBuffer tb_meta, tb_runs, tb_dir, tb_dst, tb_tables;
 };
 
 namespace gbsw {
@@ -331,7 +335,129 @@ int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
     b->stats.buckets = mode == GBSW_PLAIN ? 1 : (int32_t)b->job.buckets.size();
     b->stats.graph = ctx->ex.graph_state;
     for (const LaunchRec& L : b->launches) b->stats.main_kernel_ms = std::max(b->stats.main_kernel_ms, L.ms);
+    b->aligned = rc == GBSW_OK;
     return rc;
+}
+
+int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity, uint32_t* cigar,
+                         int64_t* cigar_offsets, int32_t* path_score, double* device_ms)
+{
+    if (!b || !sc || !cigar_offsets) return fail(GBSW_E_ARGUMENT, "null argument");
+    if (capacity < 0 || (capacity > 0 && !cigar)) return fail(GBSW_E_ARGUMENT, "bad cigar buffer");
+    if (!b->aligned) return fail(GBSW_E_ARGUMENT, "the batch has no alignment results (gbsw_batch_align first)");
+    if (sc->gap_open > sc->gap_extend)
+        return fail(GBSW_E_SCORING, "alignment paths need gap_open <= gap_extend (a gap run is one gap)");
+    gbsw_context* ctx = b->ctx;
+    GBSW_HIP_TRY(hipSetDevice(ctx->device));
+    const Job& job = b->job;
+    std::vector<int16_t> sub;
+    int32_t max_sub = 0;
+    bool p8 = false;
+    if (int rc = prepare_scoring(job, sc, sub, &max_sub, &p8)) return rc;
+    const int64_t n = job.n;
+    cigar_offsets[0] = 0;
+    if (n == 0) {
+        if (device_ms) *device_ms = 0.0;
+        return GBSW_OK;
+    }
+    hipStream_t st = ctx->ex.stream;
+    std::vector<int32_t> res((size_t)(5 * n));
+    GBSW_HIP_TRY(hipMemcpy(res.data(), b->out.ptr, sizeof(int32_t) * res.size(), hipMemcpyDeviceToHost));
+    // cells are int32: |values| stay below (rows + columns) x the largest |score|
+    int32_t big = std::max(std::abs(sc->gap_open), std::abs(sc->gap_extend));
+    for (int16_t v : sub) big = std::max<int32_t>(big, std::abs((int32_t)v));
+    // per pair: run capacity (rows + columns) and scratch region; launches of at
+    // most kTbScratch bytes of scratch (or one pair)
+    constexpr int64_t kTbScratch = (int64_t)4 << 30;
+    std::vector<int64_t> cap_off((size_t)n + 1), dir_off((size_t)n);
+    std::vector<std::pair<int64_t, int64_t>> launches; // (first pair, count)
+    int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
+    cap_off[0] = 0;
+    for (int64_t p = 0; p < n; ++p) {
+        const bool on = res[(size_t)p] > 0;
+        const int32_t m = on ? res[(size_t)(2 * n + p)] - res[(size_t)(n + p)] : 0;
+        const int32_t w = on ? res[(size_t)(4 * n + p)] - res[(size_t)(3 * n + p)] : 0;
+        if (on && (m <= 0 || w <= 0)) return fail(GBSW_E_INTERNAL, "a pair with a positive score has an empty rectangle");
+        if ((int64_t)(m + w) * big >= ((int64_t)1 << 28)) return fail(GBSW_E_RANGE, "path scores may exceed the int32 cells' range");
+        cap_off[(size_t)p + 1] = cap_off[(size_t)p] + m + w;
+        const int64_t bytes = tb_region_bytes(m, w);
+        if (chunk_bytes + bytes > kTbScratch && p > chunk_first) {
+            launches.emplace_back(chunk_first, p - chunk_first);
+            max_chunk = std::max(max_chunk, chunk_bytes);
+            chunk_first = p;
+            chunk_bytes = 0;
+        }
+        dir_off[(size_t)p] = chunk_bytes;
+        chunk_bytes += bytes;
+    }
+    launches.emplace_back(chunk_first, n - chunk_first);
+    max_chunk = std::max(max_chunk, chunk_bytes);
+    // device tables: cap_off (n+1), dir_off (n), dst_off (n+1) int64; nruns, path
+    // scores (n) int32; lut and substitution table
+    const size_t meta_bytes = sizeof(int64_t) * (size_t)(3 * n + 2) + sizeof(int32_t) * (size_t)(2 * n);
+    GBSW_HIP_TRY(b->tb_meta.reserve(meta_bytes));
+    GBSW_HIP_TRY(b->tb_runs.reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, cap_off[(size_t)n])));
+    GBSW_HIP_TRY(b->tb_dir.reserve((size_t)std::max<int64_t>(8, max_chunk)));
+    GBSW_HIP_TRY(b->tb_tables.reserve(256 + sizeof(int16_t) * sub.size()));
+    int64_t* d_cap = b->tb_meta.as<int64_t>();
+    int64_t* d_dir_off = d_cap + n + 1;
+    int64_t* d_dst_off = d_dir_off + n;
+    int32_t* d_nruns = (int32_t*)(d_dst_off + n + 1);
+    int32_t* d_score = d_nruns + n;
+    uint8_t* d_lut = b->tb_tables.as<uint8_t>();
+    int16_t* d_sub = (int16_t*)(d_lut + 256);
+    GBSW_HIP_TRY(hipMemcpyAsync(d_cap, cap_off.data(), sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st));
+    GBSW_HIP_TRY(hipMemcpyAsync(d_dir_off, dir_off.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    GBSW_HIP_TRY(hipMemcpyAsync(d_lut, b->lut, 256, hipMemcpyHostToDevice, st));
+    GBSW_HIP_TRY(hipMemcpyAsync(d_sub, sub.data(), sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, st));
+    TbArgs a{};
+    a.ref = b->ref.as<uint8_t>();
+    a.qry = b->qry.as<uint8_t>();
+    a.ref_off = b->ref_off.as<uint64_t>();
+    a.qry_off = b->qry_off.as<uint64_t>();
+    a.out = b->out.as<int32_t>();
+    a.n = n;
+    a.lut = d_lut;
+    a.sub = d_sub;
+    a.alpha = job.alpha;
+    a.gap_open = sc->gap_open;
+    a.gap_extend = sc->gap_extend;
+    a.dir = b->tb_dir.as<uint8_t>();
+    a.cap_off = d_cap;
+    a.runs = b->tb_runs.as<uint32_t>();
+    a.nruns = d_nruns;
+    a.path_score = d_score;
+    Exec& ex = ctx->ex;
+    GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st));
+    for (const auto& L : launches) {
+        a.p0 = L.first;
+        a.dir_off = d_dir_off + L.first;
+        GBSW_HIP_TRY(tb_launch(a, L.second, st));
+    }
+    GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st)); // device_ms: the path kernels
+    std::vector<int32_t> nruns((size_t)n);
+    GBSW_HIP_TRY(hipMemcpyAsync(nruns.data(), d_nruns, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+    if (path_score)
+        GBSW_HIP_TRY(hipMemcpyAsync(path_score, d_score, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+    GBSW_HIP_TRY(hipStreamSynchronize(st));
+    for (int64_t p = 0; p < n; ++p) cigar_offsets[p + 1] = cigar_offsets[p] + nruns[(size_t)p];
+    const int64_t total = cigar_offsets[n];
+    if (device_ms) {
+        float ms = 0.f;
+        GBSW_HIP_TRY(hipEventElapsedTime(&ms, ex.ev_start, ex.ev_stop));
+        *device_ms = ms;
+    }
+    if (total > capacity) {
+        return fail(GBSW_E_RANGE, "cigar capacity " + std::to_string(capacity) + " < " + std::to_string(total) +
+                                      " words needed (cigar_offsets holds the layout)");
+    }
+    GBSW_HIP_TRY(b->tb_dst.reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, total)));
+    GBSW_HIP_TRY(hipMemcpyAsync(d_dst_off, cigar_offsets, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st));
+    GBSW_HIP_TRY(tb_compact(a.runs, d_cap, d_nruns, d_dst_off, b->tb_dst.as<uint32_t>(), n, st));
+    if (total)
+        GBSW_HIP_TRY(hipMemcpyAsync(cigar, b->tb_dst.ptr, sizeof(uint32_t) * (size_t)total, hipMemcpyDeviceToHost, st));
+    GBSW_HIP_TRY(hipStreamSynchronize(st));
+    return GBSW_OK;
 }
 
 int gbsw_batch_results(gbsw_batch* b, int32_t* score, int32_t* ref_begin, int32_t* ref_end,

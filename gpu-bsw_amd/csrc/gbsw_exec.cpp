@@ -398,6 +398,12 @@ int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>
             sub[(size_t)k] = sc->matrix[k];
             mx = std::max<int>(mx, sc->matrix[k]);
         }
+        // the reverse pass may take the other sequence as rows (kernel.cpp:444-459):
+        // with an asymmetric table its maximum need not be the forward score
+        for (int r = 0; r < 24; ++r)
+            for (int c = 0; c < r; ++c)
+                if (sc->matrix[r * 24 + c] != sc->matrix[c * 24 + r])
+                    return fail(GBSW_E_SCORING, "the substitution matrix must be symmetric");
     }
     for (int16_t v : sub)
         if (v < -16000 || v > 16000) return fail(GBSW_E_SCORING, "substitution scores must lie in [-16000, 16000]");

@@ -54,4 +54,33 @@ struct GatherTile {
 hipError_t gather(const GatherTile* tiles, int ntiles, const uint8_t* accept, int32_t* part_flags,
                   hipStream_t stream);
 
+
+// gbsw_traceback.hip: alignment paths of aligned pairs.
+struct TbArgs {
+    const uint8_t* ref;
+    const uint8_t* qry;
+    const uint64_t* ref_off;
+    const uint64_t* qry_off;
+    const int32_t* out;     // results, 5 x n (score, ref_begin, ref_end, query_begin, query_end)
+    int64_t n;
+    const uint8_t* lut;     // byte -> residue code
+    const int16_t* sub;     // alpha x alpha, s(row code, column code)
+    int32_t alpha;
+    int32_t gap_open, gap_extend;
+    int64_t p0;             // first pair of the launch
+    const int64_t* dir_off; // per pair of the launch: byte offset of its region in dir
+    uint8_t* dir;           // tightness bytes and boundary columns of the launch's pairs
+    const int64_t* cap_off; // per pair: offset of its run capacity (rows + columns) in runs
+    uint32_t* runs;         // run-length ops, last run first
+    int32_t* nruns;         // per pair: runs written
+    int32_t* path_score;    // per pair: score of the path
+};
+// Scratch bytes of a pair with an m x n rectangle (0 when empty).
+int64_t tb_region_bytes(int32_t m, int32_t n);
+// Paths of pairs p0 .. p0 + count - 1 (tightness bytes, then the walk).
+hipError_t tb_launch(const TbArgs& a, int64_t count, hipStream_t stream);
+// dst[dst_off[p] ...] = runs of pair p in path order.
+hipError_t tb_compact(const uint32_t* runs, const int64_t* cap_off, const int32_t* nruns, const int64_t* dst_off,
+                      uint32_t* dst, int64_t n, hipStream_t stream);
+
 } // namespace gbsw

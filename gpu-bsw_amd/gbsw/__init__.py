@@ -26,7 +26,7 @@ FIELDS = ("score", "ref_begin", "ref_end", "query_begin", "query_end")
 GBSW_OK = 0
 ERRORS = {-1: "ARGUMENT", -2: "RESIDUE", -3: "SCORING", -4: "DEVICE", -5: "MEMORY", -6: "RANGE", -7: "INTERNAL",
           -8: "TIMEOUT"}
-E_RESIDUE, E_TIMEOUT = -2, -8
+E_RESIDUE, E_RANGE, E_TIMEOUT = -2, -6, -8
 
 
 class GbswError(RuntimeError):
@@ -113,7 +113,7 @@ EXPORTED = (
     "gbsw_version",
     "gbsw_server_config_default", "gbsw_server_create", "gbsw_server_destroy", "gbsw_server_submit",
     "gbsw_server_wait", "gbsw_server_poll", "gbsw_server_stats", "gbsw_host_alloc", "gbsw_host_free",
-    "gbsw_context_set_option",
+    "gbsw_context_set_option", "gbsw_batch_traceback",
 )
 OPT_STREAMS, OPT_F16, OPT_GEOMETRY, OPT_SPLIT, OPT_GRAPHS = 1, 2, 3, 4, 5
 # launch kinds (gbsw_launch_stat.kind): plain forward, plain reverse, batched
@@ -151,6 +151,7 @@ def load_library():
     lib.gbsw_batch_device_results.argtypes = [vp, ctypes.POINTER(vp)]
     lib.gbsw_batch_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     lib.gbsw_batch_launch_stats.argtypes = [vp, i32, ctypes.POINTER(LaunchStat)]
+    lib.gbsw_batch_traceback.argtypes = [vp, ctypes.POINTER(Scoring), i64, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     lib.gbsw_align.argtypes = [vp, ctypes.POINTER(Scoring), i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.gbsw_device_memory.argtypes = [i32, ctypes.POINTER(ctypes.c_uint64)]
     u64 = ctypes.c_uint64
@@ -281,6 +282,25 @@ class Batch:
         _check(self._lib.gbsw_batch_autotune(self._h, ctypes.byref(scoring), int(repeats), ctypes.byref(m),
                                              ctypes.byref(ms)))
         return m.value, ms.value
+
+    def traceback(self, scoring, capacity=None):
+        """gbsw_batch_traceback: alignment paths of the last align.
+
+        Returns (words, offsets, path_score, device_ms): pair k's BAM CIGAR words
+        (length << 4 | op; op 0 = M, 1 = I, 2 = D) are words[offsets[k]:offsets[k+1]].
+        """
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        ps = np.zeros(max(self.n, 1), dtype=np.int32)
+        ms = ctypes.c_double()
+        if capacity is None:
+            # rows + columns of every rectangle bound its runs
+            r = self.results()
+            capacity = int(np.sum(np.where(r["score"] > 0, (r["ref_end"] - r["ref_begin"]).astype(np.int64) +
+                                           (r["query_end"] - r["query_begin"]), 0)))
+        words = np.zeros(max(1, capacity), dtype=np.uint32)
+        _check(self._lib.gbsw_batch_traceback(self._h, ctypes.byref(scoring), int(capacity), words.ctypes.data,
+                                              off.ctypes.data, ps.ctypes.data, ctypes.byref(ms)))
+        return words[:off[-1]].copy(), off, ps[:self.n].copy(), ms.value
 
     def stats(self):
         st = Stats()
@@ -442,3 +462,11 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+CIGAR_OPS = "MID"
+
+
+def cigar_string(words):
+    """BAM CIGAR words (length << 4 | op) -> text, e.g. '12M1I30M'."""
+    return "".join("%d%s" % (int(w) >> 4, CIGAR_OPS[int(w) & 15]) for w in words)

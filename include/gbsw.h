@@ -147,6 +147,29 @@ int gbsw_batch_stats(const gbsw_batch* batch, gbsw_stats* out);
  * on the map; the batch's result arrays hold those of the last run. */
 int gbsw_batch_autotune(gbsw_batch* batch, const gbsw_scoring* scoring, int32_t repeats, int32_t* model, double* ms);
 
+/* Alignment paths (CIGAR) of the pairs of the last gbsw_batch_align.  No
+ * counterpart in the reference, which declares a traceback (traceBack,
+ * kernel.hpp:36-39) but never defines it.  Pair k's path is the best global
+ * alignment of ref[ref_begin, ref_end) against query[query_begin, query_end)
+ * under `scoring` (a run of L gap positions scores gap_open + (L-1) gap_extend;
+ * gap_open <= gap_extend is required; substitution s(row, column) with rows the
+ * longer full sequence, as in the forward pass); among optimal alignments, the
+ * one whose op sequence read from its end is lexicographically smallest with
+ * M < D < I.  Ops are BAM CIGAR words: length << 4 | op, op 0 = M (a residue
+ * pair, match or mismatch), 1 = I (a query residue against a gap), 2 = D (a ref
+ * residue against a gap).  Pairs with score 0 have no ops.
+ *   cigar_offsets: n+1 entries, always filled: pair k's words are
+ *                  cigar[cigar_offsets[k] .. cigar_offsets[k+1]).
+ *   cigar:         capacity words; when cigar_offsets[n] > capacity nothing is
+ *                  copied and GBSW_E_RANGE is returned (call again with a larger
+ *                  buffer; rows + columns per pair always suffice).
+ *   path_score:    n entries or NULL: the score of each path (equal to the pair's
+ *                  score whenever its begin and end belong to one best local
+ *                  alignment).
+ *   device_ms:     NULL or the duration of the path kernels (hipEvents). */
+int gbsw_batch_traceback(gbsw_batch* batch, const gbsw_scoring* scoring, int64_t capacity, uint32_t* cigar,
+                         int64_t* cigar_offsets, int32_t* path_score, double* device_ms);
+
 /* One kernel launch of the last gbsw_batch_align (index 0 .. stats.launches-1). */
 typedef struct {
     int32_t kind;             /* 0 plain forward, 1 plain reverse, 2 batched fwd+rev
