@@ -83,6 +83,54 @@ def cpu_baseline(w, sample_pairs, threads):
             "sample": "first %d pairs of the workload (%.3g cells), oracle fwd+rev, %.1f s" % (sub.n, sub.cells(), dt)}
 
 
+def traceback_bench(batch, scoring, res, w, args):
+    """Alignment paths (gbsw_batch_traceback) of the aligned batch: device and wall
+    time of the whole batch, in rectangle cells per second (the rectangle of a
+    pair: ref_end - ref_begin by query_end - query_begin)."""
+    on = res["score"] > 0
+    rows = (res["ref_end"] - res["ref_begin"]).astype(np.int64)
+    cols = (res["query_end"] - res["query_begin"]).astype(np.int64)
+    rect = int(np.sum(np.where(on, rows * cols, 0)))
+    cap = int(np.sum(np.where(on, rows + cols, 0)))
+    batch.traceback(scoring, capacity=cap)  # warm-up (scratch allocation)
+    dev, wall = [], []
+    for _ in range(max(1, args.traceback_steps)):
+        t = time.perf_counter()
+        words, off, ps, ms = batch.traceback(scoring, capacity=cap)
+        wall.append(time.perf_counter() - t)
+        dev.append(ms)
+    d, wl = min(dev), min(wall)
+    out = {"pairs": int(w.n), "rect_cells": rect, "device_ms": round(d, 3), "wall_ms": round(1e3 * wl, 3),
+           "device_gcups_rect": round(rect / d / 1e6, 2), "wall_gcups_rect": round(rect / wl / 1e9, 2),
+           "cigar_words": int(off[-1]), "path_score_equals_score": round(float(np.mean(ps[on] == res["score"][on])), 6)}
+    log("traceback: %d pairs, %.4g rectangle cells, device %.2f ms (%.1f GCUPS), wall %.2f ms, %d CIGAR words" %
+        (w.n, rect, d, out["device_gcups_rect"], 1e3 * wl, out["cigar_words"]))
+    if not args.no_cpu:
+        # CPU baseline: the oracle's paths on a bounded sample, checked against the
+        # device's words for the same pairs
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib as ol
+        k = min(args.traceback_cpu_pairs, w.n)
+        sub = w.subset(np.arange(k))
+        r = np.zeros(k, dtype=ol.RESULT_DTYPE)
+        for f in gbsw.FIELDS:
+            r[f] = res[f][:k]
+        nt = ol.load().gbsw_oracle_threads(args.cpu_threads)
+        t = time.perf_counter()
+        ew, eoff, eps = ol.traceback_batch(sub.ref, sub.ref_off, sub.query, sub.query_off, ol.dna_scoring(*w.scoring),
+                                           r, nthreads=nt)
+        dt = time.perf_counter() - t
+        srect = int(np.sum(np.where(on[:k], rows[:k] * cols[:k], 0)))
+        same = np.array_equal(ew, words[:off[k]]) and np.array_equal(eoff, off[:k + 1]) and np.array_equal(eps, ps[:k])
+        out["cpu_baseline"] = {"value": round(srect / dt / 1e9, 4), "unit": "GCUPS (rectangle cells)", "cores": int(nt),
+                               "kind": "port", "sample": "first %d pairs (%.3g rectangle cells), oracle paths, %.1f s" %
+                               (k, srect, dt)}
+        out["bit_exact_sample_vs_oracle"] = bool(same)
+        log("traceback cpu oracle: %.3f GCUPS on %d threads, first %d pairs bit-exact: %s" %
+            (srect / dt / 1e9, nt, k, same))
+    return out
+
+
 def time_mode(batch, scoring, mode, warmup, steps, cells):
     for _ in range(warmup):
         batch.align(scoring, mode)
@@ -204,6 +252,9 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the 10k DNA / 20k protein / long-protein lines")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default length -> geometry map")
     ap.add_argument("--long-pairs", type=int, default=50_000, help="pairs of the long-protein set")
+    ap.add_argument("--no-traceback", action="store_true", help="skip the alignment-path (CIGAR) line")
+    ap.add_argument("--traceback-steps", type=int, default=3)
+    ap.add_argument("--traceback-cpu-pairs", type=int, default=50_000)
     ap.add_argument("--serving-window", type=float, default=5.0, help="seconds per offered load")
     ap.add_argument("--serving-loads", default="0.25,0.5,0.75,1.0")
     ap.add_argument("--serving-requests", type=int, default=512, help="request pool size")
@@ -303,6 +354,13 @@ def main():
                 "kernel_ms": round(dom_ms, 3), "ops_per_cell": OPS_PER_CELL,
                 "aggregate_achieved": round(agg, 3), "aggregate_frac": round(agg / PEAK_INT_TOPS_PACKED, 4)}
 
+    tb = None
+    if not args.no_traceback:
+        tb = traceback_bench(batch, scoring, res_b, w, args)
+        if tb.get("bit_exact_sample_vs_oracle") is False:
+            log("ERROR: device alignment paths differ from the oracle's")
+            sys.exit(1)
+
     batch.close()
     ctx.close()
     configs = None
@@ -350,6 +408,7 @@ def main():
         "bit_exact_plain_vs_batched": plain is not None,
         "configs": configs,
         "serving": serving,
+        "traceback": tb,
     }
     print(json.dumps(line), flush=True)
 
