@@ -10,7 +10,7 @@
 // op sequence, read from its end, is lexicographically smallest with M < D < I.
 //
 // Three kernels per call:
-//  k_tb_dp:      one wave per pair.  Lane l owns column 64 s + l + 1 of stripe s
+//  k_tb_dp:      one wave per pair (four per workgroup).  Lane l owns column 64 s + l + 1 of stripe s
 //                and walks the rectangle's rows along anti-diagonals (row
 //                t - l + 1 at step t); the left neighbour's H and E arrive through
 //                a one-lane DPP shift, the stripe's left boundary column through
@@ -34,8 +34,7 @@ constexpr int32_t kTbNeg = -(1 << 29);
 
 enum : uint32_t {
     kHD = 1u,      // H == diagonal + s
-    kHF = 2u,      // H == F
-    kHE = 4u,      // H == E
+    kHF = 2u,      // H == F (H == E when neither of these holds)
     kFExt = 8u,    // F == F(up) + extend
     kFOpen = 16u,  // F == H(up) + open
     kEExt = 32u,   // E == E(left) + extend
@@ -76,21 +75,31 @@ __host__ __device__ __forceinline__ int64_t tb_dir_bytes(int32_t m, int32_t n)
     return (stripes * (int64_t)(m + 63) * 64 + 7) & ~(int64_t)7;
 }
 
-__global__ __launch_bounds__(64) void k_tb_dp(TbArgs a)
+constexpr int kTbWaves = 4;    // pairs (waves) per workgroup of k_tb_dp
+constexpr int kTbBlock = 256;  // steps between refills of a wave's row ring
+constexpr int kTbRing = 512;   // row codes per wave: a block's rows and the 63 before
+
+__global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count)
 {
     __shared__ uint8_t lut[256];
     __shared__ int16_t sub[kMaxAlpha * kMaxAlpha];
-    const int l = threadIdx.x;
-    for (int x = l; x < 256; x += 64) lut[x] = a.lut[x];
-    for (int x = l; x < a.alpha * a.alpha; x += 64) sub[x] = a.sub[x];
+    __shared__ uint8_t ring_all[kTbWaves][kTbRing];
+    __shared__ uint64_t bndbuf_all[kTbWaves][kTbBlock];
+    for (int x = threadIdx.x; x < 256; x += 64 * kTbWaves) lut[x] = a.lut[x];
+    for (int x = threadIdx.x; x < a.alpha * a.alpha; x += 64 * kTbWaves) sub[x] = a.sub[x];
     __syncthreads();
+    // one wave per pair from here on: no workgroup barriers below
 
-    const int64_t k = blockIdx.x;
+    const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t k = (int64_t)blockIdx.x * kTbWaves + wv;
+    if (k >= count) return;
     const int64_t p = a.p0 + k;
     if (a.out[p] <= 0) {
         if (l == 0) a.path_score[p] = 0;
         return;
     }
+    uint8_t* ring = ring_all[wv];
+    uint64_t* bndbuf = bndbuf_all[wv];
     const PairRect r = pair_rect(a, p);
     const int32_t m = r.m, n = r.n, alpha = a.alpha;
     const int32_t go = a.gap_open, ge = a.gap_extend;
@@ -104,43 +113,56 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a)
         int32_t Hup = go + (c - 1) * ge, Fup = kTbNeg; // row 0
         int32_t Hout = 0, Eout = kTbNeg, Hdiag = 0;
         uint8_t* dcol = dir + (int64_t)s * (m + 63) * 64 + l;
-        for (int32_t t = 0; t < m + 63; ++t) {
-            const int32_t i = t - l + 1;
-            int32_t Hl = from_left(Hout), El = from_left(Eout);
-            if (l == 0) {
-                if (s == 0) {
-                    Hl = go + (i - 1) * ge; // H(i, 0): a gap from the corner
-                    El = kTbNeg;
-                } else if (i >= 1 && i <= m) {
-                    const uint64_t v = __hip_atomic_load(bnd + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    Hl = (int32_t)(uint32_t)v;
-                    El = (int32_t)(uint32_t)(v >> 32);
+        for (int32_t t0 = 0; t0 < m + 63; t0 += kTbBlock) {
+            // rows [t0, t0 + 256) enter the ring as codes (rows t0 - 63 .. t0 - 1
+            // stay from the previous block), with the left boundary of those rows;
+            // lane l reads row t - l at step t, lane 0 the boundary of row t + 1
+            for (int q = 0; q < kTbBlock / 64; ++q) {
+                const int32_t x = t0 + q * 64 + l;
+                if (x < m) ring[x & (kTbRing - 1)] = lut[r.A[x]];
+                if (s > 0 && x + 1 <= m)
+                    bndbuf[q * 64 + l] = __hip_atomic_load(bnd + x + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __builtin_amdgcn_s_waitcnt(0); // this wave's ring writes land before its reads
+            const int32_t t1 = min(t0 + kTbBlock, m + 63);
+            for (int32_t t = t0; t < t1; ++t) {
+                const int32_t i = t - l + 1;
+                int32_t Hl = from_left(Hout), El = from_left(Eout);
+                if (l == 0) {
+                    if (s == 0) {
+                        Hl = go + (i - 1) * ge; // H(i, 0): a gap from the corner
+                        El = kTbNeg;
+                    } else if (i <= m) {
+                        const uint64_t v = bndbuf[t - t0];
+                        Hl = (int32_t)(uint32_t)v;
+                        El = (int32_t)(uint32_t)(v >> 32);
+                    }
                 }
+                if (i >= 1 && i <= m && vc) {
+                    const int32_t diag = i == 1 ? (c == 1 ? 0 : go + (c - 2) * ge) : Hdiag;
+                    const int code = ring[(t - l) & (kTbRing - 1)];
+                    const int32_t sv = r.swap ? sub[cb * alpha + code] : sub[code * alpha + cb];
+                    const int32_t E = max(El + ge, Hl + go);
+                    const int32_t F = max(Fup + ge, Hup + go);
+                    const int32_t D = diag + sv;
+                    const int32_t H = max(D, max(E, F));
+                    const uint32_t bits = (H == D ? kHD : 0u) | (H == F ? kHF : 0u) |
+                                          (F == Fup + ge ? kFExt : 0u) | (F == Hup + go ? kFOpen : 0u) |
+                                          (E == El + ge ? kEExt : 0u) | (E == Hl + go ? kEOpen : 0u);
+                    dcol[(int64_t)t * 64] = (uint8_t)bits;
+                    Hup = H;
+                    Fup = F;
+                    Hout = H;
+                    Eout = E;
+                    if (l == 63 && s + 1 < stripes) bnd[i] = (uint64_t)(uint32_t)H | ((uint64_t)(uint32_t)E << 32);
+                    if (i == m && c == n) a.path_score[p] = H;
+                }
+                Hdiag = Hl;
             }
-            if (i >= 1 && i <= m && vc) {
-                const int32_t diag = i == 1 ? (c == 1 ? 0 : go + (c - 2) * ge) : Hdiag;
-                const int ra = lut[r.A[i - 1]];
-                const int32_t sv = r.swap ? sub[cb * alpha + ra] : sub[ra * alpha + cb];
-                const int32_t E = max(El + ge, Hl + go);
-                const int32_t F = max(Fup + ge, Hup + go);
-                const int32_t D = diag + sv;
-                const int32_t H = max(D, max(E, F));
-                const uint32_t bits = (H == D ? kHD : 0u) | (H == F ? kHF : 0u) | (H == E ? kHE : 0u) |
-                                      (F == Fup + ge ? kFExt : 0u) | (F == Hup + go ? kFOpen : 0u) |
-                                      (E == El + ge ? kEExt : 0u) | (E == Hl + go ? kEOpen : 0u);
-                dcol[(int64_t)t * 64] = (uint8_t)bits;
-                Hup = H;
-                Fup = F;
-                Hout = H;
-                Eout = E;
-                if (l == 63 && s + 1 < stripes) bnd[i] = (uint64_t)(uint32_t)H | ((uint64_t)(uint32_t)E << 32);
-                if (i == m && c == n) a.path_score[p] = H;
-            }
-            Hdiag = Hl;
         }
-        // the boundary column is in memory before the next stripe reads it
+        // the boundary column is in memory before the next stripe's agent-scope
+        // loads read it (one wave: its own stores, complete)
         __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
     }
 }
 
@@ -228,7 +250,8 @@ int64_t tb_region_bytes(int32_t m, int32_t n)
 hipError_t tb_launch(const TbArgs& a, int64_t count, hipStream_t stream)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tb_dp, dim3((unsigned)count), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(k_tb_dp, dim3((unsigned)((count + kTbWaves - 1) / kTbWaves)), dim3(64 * kTbWaves), 0, stream, a,
+                       count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tb_walk, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, stream, a, count);
