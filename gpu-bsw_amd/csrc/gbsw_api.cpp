@@ -366,12 +366,11 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     // cells are int32: |values| stay below (rows + columns) x the largest |score|
     int32_t big = std::max(std::abs(sc->gap_open), std::abs(sc->gap_extend));
     for (int16_t v : sub) big = std::max<int32_t>(big, std::abs((int32_t)v));
-    // per pair: run capacity (rows + columns) and scratch region; launches of at
-    // most kTbScratch bytes of scratch (or one pair)
+    // per pair: run capacity (rows + columns) and scratch region
     constexpr int64_t kTbScratch = (int64_t)4 << 30;
-    std::vector<int64_t> cap_off((size_t)n + 1), dir_off((size_t)n);
-    std::vector<std::pair<int64_t, int64_t>> launches; // (first pair, count)
-    int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
+    std::vector<int64_t> cap_off((size_t)n + 1), region((size_t)n);
+    std::vector<uint8_t> cls((size_t)n);
+    int64_t cls_count[64] = {0};
     cap_off[0] = 0;
     for (int64_t p = 0; p < n; ++p) {
         const bool on = res[(size_t)p] > 0;
@@ -380,21 +379,43 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
         if (on && (m <= 0 || w <= 0)) return fail(GBSW_E_INTERNAL, "a pair with a positive score has an empty rectangle");
         if ((int64_t)(m + w) * big >= ((int64_t)1 << 28)) return fail(GBSW_E_RANGE, "path scores may exceed the int32 cells' range");
         cap_off[(size_t)p + 1] = cap_off[(size_t)p] + m + w;
-        const int64_t bytes = tb_region_bytes(m, w);
-        if (chunk_bytes + bytes > kTbScratch && p > chunk_first) {
-            launches.emplace_back(chunk_first, p - chunk_first);
+        region[(size_t)p] = tb_region_bytes(m, w);
+        // cost class: steps of the pair's DP, stripes x (rows + 63), in powers of two
+        const int64_t steps = on ? (int64_t)((w + 63) / 64) * (m + 63) : 0;
+        int c = 0;
+        while (c < 63 && ((int64_t)1 << c) < steps) ++c;
+        cls[(size_t)p] = (uint8_t)c;
+        ++cls_count[c];
+    }
+    // order: longest pairs first (their DP lasts longest; the short ones fill in
+    // behind them), by a counting sort over the cost classes
+    std::vector<int64_t> order((size_t)n);
+    {
+        int64_t start[64];
+        int64_t acc = 0;
+        for (int c = 63; c >= 0; --c) { start[c] = acc; acc += cls_count[c]; }
+        for (int64_t p = 0; p < n; ++p) order[(size_t)start[cls[(size_t)p]]++] = p;
+    }
+    // launches of at most kTbScratch bytes of scratch (or one pair), in that order
+    std::vector<int64_t> dir_off((size_t)n);
+    std::vector<std::pair<int64_t, int64_t>> launches; // (first position, count)
+    int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
+    for (int64_t x = 0; x < n; ++x) {
+        const int64_t bytes = region[(size_t)order[(size_t)x]];
+        if (chunk_bytes + bytes > kTbScratch && x > chunk_first) {
+            launches.emplace_back(chunk_first, x - chunk_first);
             max_chunk = std::max(max_chunk, chunk_bytes);
-            chunk_first = p;
+            chunk_first = x;
             chunk_bytes = 0;
         }
-        dir_off[(size_t)p] = chunk_bytes;
+        dir_off[(size_t)x] = chunk_bytes;
         chunk_bytes += bytes;
     }
     launches.emplace_back(chunk_first, n - chunk_first);
     max_chunk = std::max(max_chunk, chunk_bytes);
     // device tables: cap_off (n+1), dir_off (n), dst_off (n+1) int64; nruns, path
     // scores (n) int32; lut and substitution table
-    const size_t meta_bytes = sizeof(int64_t) * (size_t)(3 * n + 2) + sizeof(int32_t) * (size_t)(2 * n);
+    const size_t meta_bytes = sizeof(int64_t) * (size_t)(4 * n + 2) + sizeof(int32_t) * (size_t)(2 * n);
     GBSW_HIP_TRY(b->tb_meta.reserve(meta_bytes));
     GBSW_HIP_TRY(b->tb_runs.reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, cap_off[(size_t)n])));
     GBSW_HIP_TRY(b->tb_dir.reserve((size_t)std::max<int64_t>(8, max_chunk)));
@@ -402,12 +423,14 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     int64_t* d_cap = b->tb_meta.as<int64_t>();
     int64_t* d_dir_off = d_cap + n + 1;
     int64_t* d_dst_off = d_dir_off + n;
-    int32_t* d_nruns = (int32_t*)(d_dst_off + n + 1);
+    int64_t* d_order = d_dst_off + n + 1;
+    int32_t* d_nruns = (int32_t*)(d_order + n);
     int32_t* d_score = d_nruns + n;
     uint8_t* d_lut = b->tb_tables.as<uint8_t>();
     int16_t* d_sub = (int16_t*)(d_lut + 256);
     GBSW_HIP_TRY(hipMemcpyAsync(d_cap, cap_off.data(), sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st));
     GBSW_HIP_TRY(hipMemcpyAsync(d_dir_off, dir_off.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    GBSW_HIP_TRY(hipMemcpyAsync(d_order, order.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, st));
     GBSW_HIP_TRY(hipMemcpyAsync(d_lut, b->lut, 256, hipMemcpyHostToDevice, st));
     GBSW_HIP_TRY(hipMemcpyAsync(d_sub, sub.data(), sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, st));
     TbArgs a{};
@@ -427,6 +450,7 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     a.runs = b->tb_runs.as<uint32_t>();
     a.nruns = d_nruns;
     a.path_score = d_score;
+    a.order = d_order;
     Exec& ex = ctx->ex;
     GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st));
     for (const auto& L : launches) {

@@ -67,8 +67,9 @@ struct TbArgs {
     const int16_t* sub;     // alpha x alpha, s(row code, column code)
     int32_t alpha;
     int32_t gap_open, gap_extend;
-    int64_t p0;             // first pair of the launch
-    const int64_t* dir_off; // per pair of the launch: byte offset of its region in dir
+    int64_t p0;             // first position of the launch in `order`
+    const int64_t* order;   // pairs by position (longest first)
+    const int64_t* dir_off; // per position of the launch: byte offset of its region in dir
     uint8_t* dir;           // tightness bytes and boundary columns of the launch's pairs
     const int64_t* cap_off; // per pair: offset of its run capacity (rows + columns) in runs
     uint32_t* runs;         // run-length ops, last run first
@@ -77,7 +78,8 @@ struct TbArgs {
 };
 // Scratch bytes of a pair with an m x n rectangle (0 when empty).
 int64_t tb_region_bytes(int32_t m, int32_t n);
-// Paths of pairs p0 .. p0 + count - 1 (tightness bytes, then the walk).
+// Paths of the pairs at positions p0 .. p0 + count - 1 of `order` (tightness
+// bytes, then the walk).
 hipError_t tb_launch(const TbArgs& a, int64_t count, hipStream_t stream);
 // dst[dst_off[p] ...] = runs of pair p in path order.
 hipError_t tb_compact(const uint32_t* runs, const int64_t* cap_off, const int32_t* nruns, const int64_t* dst_off,

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t k = (int64_t)blockIdx.x * kTbWaves + wv;
     if (k >= count) return;
-    const int64_t p = a.p0 + k;
+    const int64_t p = a.order[a.p0 + k];
     if (a.out[p] <= 0) {
         if (l == 0) a.path_score[p] = 0;
         return;
@@ -127,17 +127,22 @@ __global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count
             const int32_t t1 = min(t0 + kTbBlock, m + 63);
             for (int32_t t = t0; t < t1; ++t) {
                 const int32_t i = t - l + 1;
-                int32_t Hl = from_left(Hout), El = from_left(Eout);
-                if (l == 0) {
-                    if (s == 0) {
-                        Hl = go + (i - 1) * ge; // H(i, 0): a gap from the corner
-                        El = kTbNeg;
-                    } else if (i <= m) {
-                        const uint64_t v = bndbuf[t - t0];
-                        Hl = (int32_t)(uint32_t)v;
-                        El = (int32_t)(uint32_t)(v >> 32);
-                    }
+                // left boundary of lane 0's row t + 1 (a uniform branch and a
+                // broadcast read; rows past the rectangle read unused entries)
+                int32_t bH, bE;
+                if (s == 0) {
+                    bH = go + t * ge; // H(t + 1, 0): a gap from the corner
+                    bE = kTbNeg;
+                } else {
+                    const uint64_t v = bndbuf[t - t0];
+                    bH = (int32_t)(uint32_t)v;
+                    bE = (int32_t)(uint32_t)(v >> 32);
                 }
+                // the shifts run in every lane (a DPP read from a lane outside the
+                // exec mask would return the fill value)
+                const int32_t sH = from_left(Hout), sE = from_left(Eout);
+                const int32_t Hl = l == 0 ? bH : sH;
+                const int32_t El = l == 0 ? bE : sE;
                 if (i >= 1 && i <= m && vc) {
                     const int32_t diag = i == 1 ? (c == 1 ? 0 : go + (c - 2) * ge) : Hdiag;
                     const int code = ring[(t - l) & (kTbRing - 1)];
@@ -176,7 +181,7 @@ __global__ __launch_bounds__(64) void k_tb_walk(TbArgs a, int64_t count)
 {
     const int64_t k = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (k >= count) return;
-    const int64_t p = a.p0 + k;
+    const int64_t p = a.order[a.p0 + k];
     if (a.out[p] <= 0) {
         a.nruns[p] = 0;
         return;
