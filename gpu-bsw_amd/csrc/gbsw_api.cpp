@@ -50,6 +50,11 @@ struct gbsw_batch {
     bool aligned = false;  // results of a gbsw_batch_align are in `out`
     // alignment paths: per-pair tables, run buffers, tightness scratch
     DeviceBuffer tb_meta, tb_runs, tb_dir, tb_dst, tb_tables;
+    std::vector<hipEvent_t> tb_events;
+    ~gbsw_batch()
+    {
+        for (hipEvent_t e : tb_events) (void)hipEventDestroy(e);
+    }
 };
 
 namespace gbsw {
@@ -418,7 +423,9 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     const size_t meta_bytes = sizeof(int64_t) * (size_t)(4 * n + 2) + sizeof(int32_t) * (size_t)(2 * n);
     GBSW_HIP_TRY(b->tb_meta.reserve(meta_bytes));
     GBSW_HIP_TRY(b->tb_runs.reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, cap_off[(size_t)n])));
-    GBSW_HIP_TRY(b->tb_dir.reserve((size_t)std::max<int64_t>(8, max_chunk)));
+    // two scratch halves: the walk of one launch overlaps the DP of the next
+    const size_t half = ((size_t)std::max<int64_t>(8, max_chunk) + 255) & ~(size_t)255;
+    GBSW_HIP_TRY(b->tb_dir.reserve(2 * half));
     GBSW_HIP_TRY(b->tb_tables.reserve(256 + sizeof(int16_t) * sub.size()));
     int64_t* d_cap = b->tb_meta.as<int64_t>();
     int64_t* d_dir_off = d_cap + n + 1;
@@ -453,11 +460,28 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     a.order = d_order;
     Exec& ex = ctx->ex;
     GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st));
-    for (const auto& L : launches) {
-        a.p0 = L.first;
-        a.dir_off = d_dir_off + L.first;
-        GBSW_HIP_TRY(tb_launch(a, L.second, st));
+    // DP kernels on the context stream, walks on an auxiliary one; launch c's walk
+    // waits for its DP, and launch c + 2's DP (same scratch half) for that walk
+    hipStream_t sw = ex.aux[0];
+    while (b->tb_events.size() < 2 * launches.size()) {
+        hipEvent_t e;
+        GBSW_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->tb_events.push_back(e);
     }
+    GBSW_HIP_TRY(hipEventRecord(b->tb_events[0], st));
+    GBSW_HIP_TRY(hipStreamWaitEvent(sw, b->tb_events[0], 0)); // the tables are uploaded
+    for (size_t c = 0; c < launches.size(); ++c) {
+        a.p0 = launches[c].first;
+        a.dir_off = d_dir_off + launches[c].first;
+        a.dir = b->tb_dir.as<uint8_t>() + (c & 1) * half;
+        if (c >= 2) GBSW_HIP_TRY(hipStreamWaitEvent(st, b->tb_events[2 * (c - 2) + 1], 0));
+        GBSW_HIP_TRY(tb_launch_dp(a, launches[c].second, st));
+        GBSW_HIP_TRY(hipEventRecord(b->tb_events[2 * c], st));
+        GBSW_HIP_TRY(hipStreamWaitEvent(sw, b->tb_events[2 * c], 0));
+        GBSW_HIP_TRY(tb_launch_walk(a, launches[c].second, sw));
+        GBSW_HIP_TRY(hipEventRecord(b->tb_events[2 * c + 1], sw));
+    }
+    GBSW_HIP_TRY(hipStreamWaitEvent(st, b->tb_events[2 * (launches.size() - 1) + 1], 0));
     GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st)); // device_ms: the path kernels
     std::vector<int32_t> nruns((size_t)n);
     GBSW_HIP_TRY(hipMemcpyAsync(nruns.data(), d_nruns, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));

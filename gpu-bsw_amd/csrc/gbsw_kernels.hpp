@@ -78,9 +78,10 @@ struct TbArgs {
 };
 // Scratch bytes of a pair with an m x n rectangle (0 when empty).
 int64_t tb_region_bytes(int32_t m, int32_t n);
-// Paths of the pairs at positions p0 .. p0 + count - 1 of `order` (tightness
-// bytes, then the walk).
-hipError_t tb_launch(const TbArgs& a, int64_t count, hipStream_t stream);
+// Tightness bytes of the pairs at positions p0 .. p0 + count - 1 of `order`.
+hipError_t tb_launch_dp(const TbArgs& a, int64_t count, hipStream_t stream);
+// Their paths (run-length ops, last run first), from the bytes tb_launch_dp wrote.
+hipError_t tb_launch_walk(const TbArgs& a, int64_t count, hipStream_t stream);
 // dst[dst_off[p] ...] = runs of pair p in path order.
 hipError_t tb_compact(const uint32_t* runs, const int64_t* cap_off, const int32_t* nruns, const int64_t* dst_off,
                       uint32_t* dst, int64_t n, hipStream_t stream);

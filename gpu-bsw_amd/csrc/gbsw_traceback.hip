@@ -252,13 +252,17 @@ int64_t tb_region_bytes(int32_t m, int32_t n)
     return tb_dir_bytes(m, n) + bnd;
 }
 
-hipError_t tb_launch(const TbArgs& a, int64_t count, hipStream_t stream)
+hipError_t tb_launch_dp(const TbArgs& a, int64_t count, hipStream_t stream)
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tb_dp, dim3((unsigned)((count + kTbWaves - 1) / kTbWaves)), dim3(64 * kTbWaves), 0, stream, a,
                        count);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t tb_launch_walk(const TbArgs& a, int64_t count, hipStream_t stream)
+{
+    if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tb_walk, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, stream, a, count);
     return hipGetLastError();
 }
