@@ -258,7 +258,7 @@ def main():
     ap.add_argument("--serving-window", type=float, default=5.0, help="seconds per offered load")
     ap.add_argument("--serving-loads", default="0.25,0.5,0.75,1.0")
     ap.add_argument("--serving-requests", type=int, default=512, help="request pool size")
-    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-slots", type=int, default=2)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (rocprof FETCH/WRITE calibration)")
     args = ap.parse_args()

@@ -690,7 +690,7 @@ void gbsw_server_config_default(gbsw_server_config* cfg)
 {
     if (!cfg) return;
     std::memset(cfg, 0, sizeof(*cfg));
-    cfg->slots = 4;
+    cfg->slots = 2;
     cfg->pairs_per_slot = 262144;
     cfg->residues_per_slot = (int64_t)256 << 20;
     cfg->boundary_bytes_per_slot = (int64_t)1 << 30;

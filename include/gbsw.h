@@ -212,7 +212,7 @@ int gbsw_device_memory(int32_t device, uint64_t* bytes);
 typedef struct gbsw_server gbsw_server;
 
 typedef struct {
-    int32_t slots;                   /* micro-batches in flight (default 4)                 */
+    int32_t slots;                   /* micro-batches in flight (default 2)                 */
     int32_t pairs_per_slot;          /* pair capacity of one micro-batch (default 262144)   */
     int64_t residues_per_slot;       /* ref+query bytes of one micro-batch (default 256 MiB) */
     int64_t boundary_bytes_per_slot; /* strip-boundary scratch of one micro-batch (default 1 GiB) */
