@@ -43,10 +43,10 @@ enum : uint32_t {
 
 enum : uint32_t { kOpM = 0, kOpI = 1, kOpD = 2 }; // BAM CIGAR op codes
 
-// value of the lane one below (wave_shr:1); lane 0 receives 0
+// value of the lane one below (wave_shr:1); lane 0 receives 0 (bound_ctrl)
 __device__ __forceinline__ int32_t from_left(int32_t v)
 {
-    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
 }
 
 struct PairRect {
@@ -90,7 +90,8 @@ __global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count
     __syncthreads();
     // one wave per pair from here on: no workgroup barriers below
 
-    const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave index in a scalar register: the pair's sizes and addresses stay scalar
+    const int l = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t k = (int64_t)blockIdx.x * kTbWaves + wv;
     if (k >= count) return;
     const int64_t p = a.order[a.p0 + k];
@@ -112,7 +113,9 @@ __global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count
         const int cb = vc ? lut[r.B[c - 1]] : 0;
         int32_t Hup = go + (c - 1) * ge, Fup = kTbNeg; // row 0
         int32_t Hout = 0, Eout = kTbNeg, Hdiag = 0;
-        uint8_t* dcol = dir + (int64_t)s * (m + 63) * 64 + l;
+        // byte offset of this lane's cell at step t: dcol + 64 t (a pair's region
+        // is below 4 GiB)
+        const uint32_t dcol = (uint32_t)((int64_t)s * (m + 63) * 64) + (uint32_t)l;
         for (int32_t t0 = 0; t0 < m + 63; t0 += kTbBlock) {
             // rows [t0, t0 + 256) enter the ring as codes (rows t0 - 63 .. t0 - 1
             // stay from the previous block), with the left boundary of those rows;
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count
                     const uint32_t bits = (H == D ? kHD : 0u) | (H == F ? kHF : 0u) |
                                           (F == Fup + ge ? kFExt : 0u) | (F == Hup + go ? kFOpen : 0u) |
                                           (E == El + ge ? kEExt : 0u) | (E == Hl + go ? kEOpen : 0u);
-                    dcol[(int64_t)t * 64] = (uint8_t)bits;
+                    dir[dcol + (uint32_t)t * 64u] = (uint8_t)bits;
                     Hup = H;
                     Fup = F;
                     Hout = H;
