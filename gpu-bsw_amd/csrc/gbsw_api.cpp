@@ -30,10 +30,12 @@ const char kAaOrder[] = "ARNDCQEGHILKMFPSTWYVBZX*";
 
 } // namespace
 
+struct gbsw_batch;
 struct gbsw_context {
     int device = 0;
     int num_cu = 0;
     Exec ex;
+    gbsw_batch* oneshot = nullptr; // gbsw_align's batch, reused across calls
 };
 
 struct gbsw_batch {
@@ -110,6 +112,117 @@ int alphabet_from_mask(int32_t alphabet, const uint32_t* mask, uint8_t* lut)
 
 } // namespace gbsw
 
+namespace gbsw {
+
+// Upload pairs into batch b (new, or one reused: its device buffers only grow).
+static int batch_fill(gbsw_batch* b, gbsw_context* ctx, int32_t alphabet, int64_t n, const char* ref,
+               const uint64_t* ref_off, const char* query, const uint64_t* query_off)
+{
+    if (!ctx || n < 0 || (n > 0 && (!ref_off || !query_off)))
+        return fail(GBSW_E_ARGUMENT, "null argument or negative pair count");
+    if (alphabet != GBSW_DNA && alphabet != GBSW_PROTEIN) return fail(GBSW_E_ARGUMENT, "unknown alphabet");
+    if (n > (int64_t)0x7FFFFFFF) return fail(GBSW_E_RANGE, "more than 2^31-1 pairs in one batch");
+    const uint64_t ref_bytes = n ? ref_off[n] - ref_off[0] : 0;
+    const uint64_t qry_bytes = n ? query_off[n] - query_off[0] : 0;
+    if ((ref_bytes && !ref) || (qry_bytes && !query)) return fail(GBSW_E_ARGUMENT, "sequence buffer is null");
+    GBSW_HIP_TRY(hipSetDevice(ctx->device));
+
+    b->ctx = ctx;
+    b->job = Job{};
+    b->graph.reset();
+    b->aligned = false;
+    b->stats = gbsw_stats{};
+    b->launches.clear();
+    Job& job = b->job;
+    job.alphabet = alphabet;
+    job.n = n;
+    b->len_ref.resize((size_t)n);
+    b->len_qry.resize((size_t)n);
+    std::vector<uint64_t> ro((size_t)n + 1), qo((size_t)n + 1);
+    for (int64_t p = 0; p <= n; ++p) {
+        ro[(size_t)p] = n ? ref_off[p] - ref_off[0] : 0;
+        qo[(size_t)p] = n ? query_off[p] - query_off[0] : 0;
+    }
+    for (int64_t p = 0; p < n; ++p) {
+        if (ro[(size_t)p + 1] < ro[(size_t)p] || qo[(size_t)p + 1] < qo[(size_t)p]) {
+            return fail(GBSW_E_ARGUMENT, "offsets must be non-decreasing");
+        }
+        const uint64_t lr = ro[(size_t)p + 1] - ro[(size_t)p], lq = qo[(size_t)p + 1] - qo[(size_t)p];
+        if (lr > 65000 || lq > 65000) {
+            return fail(GBSW_E_RANGE, "sequences longer than 65000 residues are not supported");
+        }
+        b->len_ref[(size_t)p] = (int32_t)lr;
+        b->len_qry[(size_t)p] = (int32_t)lq;
+        job.max_rows = std::max<int32_t>(job.max_rows, (int32_t)std::max(lr, lq));
+        job.max_cols = std::max<int32_t>(job.max_cols, (int32_t)std::min(lr, lq));
+        job.cells += (int64_t)(lr * lq);
+    }
+    const uint8_t* rp = (const uint8_t*)ref + (n ? ref_off[0] : 0);
+    const uint8_t* qp = (const uint8_t*)query + (n ? query_off[0] : 0);
+    job.lut_host = b->lut;
+    job.kmodel = ctx->ex.kmodel;
+    size_t sort_bytes = 0;
+    int rc = sort_scratch_bytes(n, &sort_bytes);
+    if (rc) return rc;
+    const size_t np = (size_t)std::max<int64_t>(n, 1);
+    hipError_t e = b->ref.reserve(std::max<uint64_t>(ref_bytes, 16));
+    if (e == hipSuccess) e = b->qry.reserve(std::max<uint64_t>(qry_bytes, 16));
+    if (e == hipSuccess) e = b->ref_off.reserve(sizeof(uint64_t) * (np + 1));
+    if (e == hipSuccess) e = b->qry_off.reserve(sizeof(uint64_t) * (np + 1));
+    if (e == hipSuccess) e = b->out.reserve(sizeof(int32_t) * 5 * np);
+    if (e == hipSuccess) e = b->keys.reserve(std::max<size_t>(4 * np, 32));
+    if (e == hipSuccess) e = b->keys_sorted.reserve(4 * np);
+    if (e == hipSuccess) e = b->vals.reserve(4 * np);
+    if (e == hipSuccess) e = b->order.reserve(4 * np);
+    if (e == hipSuccess) e = b->sort_temp.reserve(sort_bytes);
+    if (e == hipSuccess) e = b->bucket_of_len.reserve(65536);
+    if (e == hipSuccess && ref_bytes) e = hipMemcpy(b->ref.ptr, rp, ref_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && qry_bytes) e = hipMemcpy(b->qry.ptr, qp, qry_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->ref_off.ptr, ro.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->qry_off.ptr, qo.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->out.ptr, 0, sizeof(int32_t) * 5 * np);
+    // The residue codes need the set of byte values in the batch (every distinct
+    // DNA byte is a code; protein bytes must be < 91): a scan of the uploaded
+    // bytes on the device instead of a pass over them on the host.
+    uint32_t seen[8] = {0};
+    if (e == hipSuccess) {
+        hipStream_t st = ctx->ex.stream;
+        uint32_t* dmask = b->keys.as<uint32_t>();
+        e = zero_u32(dmask, 8, st);
+        if (e == hipSuccess) e = byte_presence(b->ref.as<uint8_t>(), ref_bytes, dmask, st);
+        if (e == hipSuccess) e = byte_presence(b->qry.as<uint8_t>(), qry_bytes, dmask, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(seen, dmask, sizeof(seen), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? GBSW_E_MEMORY : GBSW_E_DEVICE,
+                    std::string("batch upload: ") + hipGetErrorString(e));
+    {
+        const int A = alphabet_from_mask(alphabet, seen, b->lut);
+        if (A < 0) return A;
+        job.alpha = A;
+    }
+    count_buckets(job, b->len_ref.data(), b->len_qry.data());
+    e = hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, job.kmodel).data(), 65536, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(GBSW_E_DEVICE, std::string("batch upload: ") + hipGetErrorString(e));
+    job.ref = b->ref.as<uint8_t>();
+    job.qry = b->qry.as<uint8_t>();
+    job.ref_off = b->ref_off.as<uint64_t>();
+    job.qry_off = b->qry_off.as<uint64_t>();
+    job.out = b->out.as<int32_t>();
+    job.bucket_of_len = b->bucket_of_len.as<uint8_t>();
+    job.keys = b->keys.as<uint32_t>();
+    job.keys_sorted = b->keys_sorted.as<uint32_t>();
+    job.vals = b->vals.as<int32_t>();
+    job.order = b->order.as<int32_t>();
+    job.sort_temp = b->sort_temp.ptr;
+    job.sort_temp_bytes = sort_bytes;
+    b->stats.cells = job.cells;
+    return GBSW_OK;
+}
+
+} // namespace gbsw
+
 extern "C" {
 
 const char* gbsw_last_error(void) { return gbsw::last_error_text(); }
@@ -182,6 +295,7 @@ int gbsw_context_destroy(gbsw_context* ctx)
 {
     if (!ctx) return GBSW_OK;
     (void)hipSetDevice(ctx->device);
+    delete ctx->oneshot;
     ctx->ex.destroy();
     delete ctx;
     return GBSW_OK;
@@ -191,118 +305,16 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
                       const uint64_t* ref_off, const char* query, const uint64_t* query_off,
                       gbsw_batch** out)
 {
-    if (!ctx || !out || n < 0 || (n > 0 && (!ref_off || !query_off)))
-        return fail(GBSW_E_ARGUMENT, "null argument or negative pair count");
-    if (alphabet != GBSW_DNA && alphabet != GBSW_PROTEIN) return fail(GBSW_E_ARGUMENT, "unknown alphabet");
-    if (n > (int64_t)0x7FFFFFFF) return fail(GBSW_E_RANGE, "more than 2^31-1 pairs in one batch");
+    if (!out) return fail(GBSW_E_ARGUMENT, "out is null");
     *out = nullptr;
-    const uint64_t ref_bytes = n ? ref_off[n] - ref_off[0] : 0;
-    const uint64_t qry_bytes = n ? query_off[n] - query_off[0] : 0;
-    if ((ref_bytes && !ref) || (qry_bytes && !query)) return fail(GBSW_E_ARGUMENT, "sequence buffer is null");
-    GBSW_HIP_TRY(hipSetDevice(ctx->device));
-
     gbsw_batch* b = new gbsw_batch();
-    b->ctx = ctx;
-    Job& job = b->job;
-    job.alphabet = alphabet;
-    job.n = n;
-    b->len_ref.resize((size_t)n);
-    b->len_qry.resize((size_t)n);
-    std::vector<uint64_t> ro((size_t)n + 1), qo((size_t)n + 1);
-    for (int64_t p = 0; p <= n; ++p) {
-        ro[(size_t)p] = n ? ref_off[p] - ref_off[0] : 0;
-        qo[(size_t)p] = n ? query_off[p] - query_off[0] : 0;
-    }
-    for (int64_t p = 0; p < n; ++p) {
-        if (ro[(size_t)p + 1] < ro[(size_t)p] || qo[(size_t)p + 1] < qo[(size_t)p]) {
-            delete b;
-            return fail(GBSW_E_ARGUMENT, "offsets must be non-decreasing");
-        }
-        const uint64_t lr = ro[(size_t)p + 1] - ro[(size_t)p], lq = qo[(size_t)p + 1] - qo[(size_t)p];
-        if (lr > 65000 || lq > 65000) {
-            delete b;
-            return fail(GBSW_E_RANGE, "sequences longer than 65000 residues are not supported");
-        }
-        b->len_ref[(size_t)p] = (int32_t)lr;
-        b->len_qry[(size_t)p] = (int32_t)lq;
-        job.max_rows = std::max<int32_t>(job.max_rows, (int32_t)std::max(lr, lq));
-        job.max_cols = std::max<int32_t>(job.max_cols, (int32_t)std::min(lr, lq));
-        job.cells += (int64_t)(lr * lq);
-    }
-    const uint8_t* rp = (const uint8_t*)ref + (n ? ref_off[0] : 0);
-    const uint8_t* qp = (const uint8_t*)query + (n ? query_off[0] : 0);
-    job.lut_host = b->lut;
-    job.kmodel = ctx->ex.kmodel;
-    size_t sort_bytes = 0;
-    int rc = sort_scratch_bytes(n, &sort_bytes);
+    const int rc = batch_fill(b, ctx, alphabet, n, ref, ref_off, query, query_off);
     if (rc) {
+        const std::string err = gbsw_last_error();
         delete b;
+        set_error(err);
         return rc;
     }
-    const size_t np = (size_t)std::max<int64_t>(n, 1);
-    hipError_t e = b->ref.reserve(std::max<uint64_t>(ref_bytes, 16));
-    if (e == hipSuccess) e = b->qry.reserve(std::max<uint64_t>(qry_bytes, 16));
-    if (e == hipSuccess) e = b->ref_off.reserve(sizeof(uint64_t) * (np + 1));
-    if (e == hipSuccess) e = b->qry_off.reserve(sizeof(uint64_t) * (np + 1));
-    if (e == hipSuccess) e = b->out.reserve(sizeof(int32_t) * 5 * np);
-    if (e == hipSuccess) e = b->keys.reserve(std::max<size_t>(4 * np, 32));
-    if (e == hipSuccess) e = b->keys_sorted.reserve(4 * np);
-    if (e == hipSuccess) e = b->vals.reserve(4 * np);
-    if (e == hipSuccess) e = b->order.reserve(4 * np);
-    if (e == hipSuccess) e = b->sort_temp.reserve(sort_bytes);
-    if (e == hipSuccess) e = b->bucket_of_len.reserve(65536);
-    if (e == hipSuccess && ref_bytes) e = hipMemcpy(b->ref.ptr, rp, ref_bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess && qry_bytes) e = hipMemcpy(b->qry.ptr, qp, qry_bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->ref_off.ptr, ro.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->qry_off.ptr, qo.data(), sizeof(uint64_t) * (size_t)(n + 1), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(b->out.ptr, 0, sizeof(int32_t) * 5 * np);
-    // The residue codes need the set of byte values in the batch (every distinct
-    // DNA byte is a code; protein bytes must be < 91): a scan of the uploaded
-    // bytes on the device instead of a pass over them on the host.
-    uint32_t seen[8] = {0};
-    if (e == hipSuccess) {
-        hipStream_t st = ctx->ex.stream;
-        uint32_t* dmask = b->keys.as<uint32_t>();
-        e = zero_u32(dmask, 8, st);
-        if (e == hipSuccess) e = byte_presence(b->ref.as<uint8_t>(), ref_bytes, dmask, st);
-        if (e == hipSuccess) e = byte_presence(b->qry.as<uint8_t>(), qry_bytes, dmask, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(seen, dmask, sizeof(seen), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-    }
-    if (e != hipSuccess) {
-        delete b;
-        return fail(e == hipErrorOutOfMemory ? GBSW_E_MEMORY : GBSW_E_DEVICE,
-                    std::string("batch upload: ") + hipGetErrorString(e));
-    }
-    {
-        const int A = alphabet_from_mask(alphabet, seen, b->lut);
-        if (A < 0) {
-            const std::string err = gbsw_last_error();
-            delete b;
-            set_error(err);
-            return A;
-        }
-        job.alpha = A;
-    }
-    count_buckets(job, b->len_ref.data(), b->len_qry.data());
-    e = hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, job.kmodel).data(), 65536, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        delete b;
-        return fail(GBSW_E_DEVICE, std::string("batch upload: ") + hipGetErrorString(e));
-    }
-    job.ref = b->ref.as<uint8_t>();
-    job.qry = b->qry.as<uint8_t>();
-    job.ref_off = b->ref_off.as<uint64_t>();
-    job.qry_off = b->qry_off.as<uint64_t>();
-    job.out = b->out.as<int32_t>();
-    job.bucket_of_len = b->bucket_of_len.as<uint8_t>();
-    job.keys = b->keys.as<uint32_t>();
-    job.keys_sorted = b->keys_sorted.as<uint32_t>();
-    job.vals = b->vals.as<int32_t>();
-    job.order = b->order.as<int32_t>();
-    job.sort_temp = b->sort_temp.ptr;
-    job.sort_temp_bytes = sort_bytes;
-    b->stats.cells = job.cells;
     *out = b;
     return GBSW_OK;
 }
@@ -598,16 +610,18 @@ int gbsw_align(gbsw_context* ctx, const gbsw_scoring* sc, int32_t mode, int64_t 
                const uint64_t* ref_off, const char* query, const uint64_t* query_off, int32_t* score,
                int32_t* ref_begin, int32_t* ref_end, int32_t* query_begin, int32_t* query_end)
 {
-    if (!sc) return fail(GBSW_E_ARGUMENT, "scoring is null");
-    gbsw_batch* b = nullptr;
-    int rc = gbsw_batch_create(ctx, sc->alphabet, n, ref, ref_off, query, query_off, &b);
-    if (rc) return rc;
-    b->one_shot = true;
-    rc = gbsw_batch_align(b, sc, mode);
+    if (!sc || !ctx) return fail(GBSW_E_ARGUMENT, "null argument");
+    // the context's one-shot batch: its device buffers are kept and only grow, so
+    // repeated calls allocate and free nothing (a free would also wait for the
+    // whole device)
+    if (!ctx->oneshot) {
+        ctx->oneshot = new gbsw_batch();
+        ctx->oneshot->one_shot = true;
+    }
+    gbsw_batch* b = ctx->oneshot;
+    int rc = batch_fill(b, ctx, sc->alphabet, n, ref, ref_off, query, query_off);
+    if (!rc) rc = gbsw_batch_align(b, sc, mode);
     if (!rc) rc = gbsw_batch_results(b, score, ref_begin, ref_end, query_begin, query_end);
-    const std::string err = gbsw_last_error();
-    gbsw_batch_destroy(b);
-    set_error(err);
     return rc;
 }
 

@@ -5,7 +5,9 @@
 // "Total ..." summary lines on stdout.  The strings are packed into pinned
 // staging (as the reference packs into pinned buffers, driver.cpp:113-161, here
 // on several threads and kept across calls), then device transfer and both
-// passes go through gbsw_align.
+// passes go through gbsw_align.  Calls of 131,072 pairs or more run as four
+// sub-batches alternating between two contexts: while one sub-batch aligns, the
+// next is packed and uploaded.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -25,18 +27,23 @@
 namespace {
 
 thread_local int g_status = 0;
+// Large calls run as sub-batches on two contexts (two sets of streams), so the
+// upload of one overlaps the alignment of the other while the host packs the next.
+constexpr int kLanes = 2;
+constexpr size_t kPipelinePairs = 131072; // below this, one sub-batch
+constexpr int kSubBatches = 4;
 std::mutex g_ctx_mu;
-gbsw_context* g_ctx = nullptr;
+gbsw_context* g_ctx[kLanes] = {};
 
-gbsw_context* shared_context(int* rc)
+gbsw_context* shared_context(int lane, int* rc)
 {
     std::lock_guard<std::mutex> lk(g_ctx_mu);
-    if (g_ctx) { *rc = GBSW_OK; return g_ctx; }
+    if (g_ctx[lane]) { *rc = GBSW_OK; return g_ctx[lane]; }
     int dev = 0;
     if (const char* e = std::getenv("GBSW_DEVICE")) dev = std::atoi(e);
     else (void)hipGetDevice(&dev);
-    *rc = gbsw_context_create(dev, &g_ctx);
-    return *rc == GBSW_OK ? g_ctx : nullptr;
+    *rc = gbsw_context_create(dev, &g_ctx[lane]);
+    return *rc == GBSW_OK ? g_ctx[lane] : nullptr;
 }
 
 int32_t driver_mode()
@@ -76,25 +83,26 @@ struct Staging {
         return true;
     }
 };
-std::mutex g_stage_mu; // one staging pair, held for a whole call
-Staging g_stage_ref, g_stage_qry;
+std::mutex g_stage_mu; // the staging sets, held for a whole call
+Staging g_stage_ref[kLanes], g_stage_qry[kLanes];
 
-void offsets(const std::vector<std::string>& v, std::vector<uint64_t>& off, size_t n)
+// Offsets of strings v[first .. first + n) relative to the first.
+void offsets(const std::vector<std::string>& v, size_t first, std::vector<uint64_t>& off, size_t n)
 {
     off.resize(n + 1);
     off[0] = 0;
-    for (size_t k = 0; k < n; ++k) off[k + 1] = off[k] + v[k].size();
+    for (size_t k = 0; k < n; ++k) off[k + 1] = off[k] + v[first + k].size();
 }
 
-// The strings at their offsets in dst, copied by up to 8 threads (one for small
-// inputs).
-void pack(const std::vector<std::string>& v, const std::vector<uint64_t>& off, size_t n, char* dst)
+// The strings v[first .. first + n) at their offsets in dst, copied by up to 8
+// threads (one for small inputs).
+void pack(const std::vector<std::string>& v, size_t first, const std::vector<uint64_t>& off, size_t n, char* dst)
 {
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const unsigned T = off[n] < ((uint64_t)64 << 20) ? 1u : std::min(8u, hw);
+    const unsigned T = off[n] < ((uint64_t)16 << 20) ? 1u : std::min(8u, hw);
     auto part = [&](unsigned t) {
         for (size_t k = n * t / T; k < n * (t + 1) / T; ++k)
-            if (!v[k].empty()) std::memcpy(dst + off[k], v[k].data(), v[k].size());
+            if (!v[first + k].empty()) std::memcpy(dst + off[k], v[first + k].data(), v[first + k].size());
     };
     std::vector<std::thread> th;
     for (unsigned t = 1; t < T; ++t) th.emplace_back(part, t);
@@ -102,9 +110,9 @@ void pack(const std::vector<std::string>& v, const std::vector<uint64_t>& off, s
     for (std::thread& x : th) x.join();
 }
 
-[[noreturn]] void die(int rc)
+[[noreturn]] void die(int rc, const std::string& msg = std::string())
 {
-    std::fprintf(stderr, "GPUassert: %s (gbsw status %d)\n", gbsw_last_error(), rc);
+    std::fprintf(stderr, "GPUassert: %s (gbsw status %d)\n", msg.empty() ? gbsw_last_error() : msg.c_str(), rc);
     std::exit(1);
 }
 
@@ -133,26 +141,54 @@ void run(std::vector<std::string>& reads, std::vector<std::string>& contigs,
     }
     const auto t0 = std::chrono::high_resolution_clock::now();
     int rc = GBSW_OK;
-    gbsw_context* ctx = shared_context(&rc);
-    std::vector<uint64_t> ro, qo;
+    std::string err; // a sub-batch's error message (set on its own thread)
     std::vector<int32_t> out(5 * (n ? n : 1));
-    if (rc == GBSW_OK) {
+    {
         std::lock_guard<std::mutex> lk(g_stage_mu);
-        offsets(contigs, ro, n);
-        offsets(reads, qo, n);
-        if (!g_stage_ref.reserve(ro[n] + 1) || !g_stage_qry.reserve(qo[n] + 1)) {
-            rc = GBSW_E_MEMORY;
-        } else {
-            pack(contigs, ro, n, g_stage_ref.p);
-            pack(reads, qo, n, g_stage_qry.p);
-            rc = gbsw_align(ctx, &sc, driver_mode(), (int64_t)n, g_stage_ref.p, ro.data(), g_stage_qry.p, qo.data(),
-                            out.data(), out.data() + n, out.data() + 2 * n, out.data() + 3 * n, out.data() + 4 * n);
+        // sub-batch k: pairs [n k / S, n (k+1) / S) on lane k % 2; the lane's
+        // previous sub-batch has finished before its staging is refilled
+        const int S = n >= kPipelinePairs ? kSubBatches : 1;
+        std::vector<uint64_t> ro[kLanes], qo[kLanes];
+        std::thread worker[kLanes];
+        int lane_rc[kLanes] = {GBSW_OK, GBSW_OK};
+        std::string lane_err[kLanes];
+        const int32_t mode = driver_mode();
+        for (int k = 0; k < S && rc == GBSW_OK; ++k) {
+            const int lane = k % kLanes;
+            if (worker[lane].joinable()) worker[lane].join();
+            if (lane_rc[lane] != GBSW_OK) break;
+            gbsw_context* ctx = shared_context(lane, &rc);
+            if (rc != GBSW_OK) break;
+            const size_t p0 = n * (size_t)k / (size_t)S, m = n * (size_t)(k + 1) / (size_t)S - p0;
+            offsets(contigs, p0, ro[lane], m);
+            offsets(reads, p0, qo[lane], m);
+            Staging& sr = g_stage_ref[lane];
+            Staging& sq = g_stage_qry[lane];
+            if (!sr.reserve(ro[lane][m] + 1) || !sq.reserve(qo[lane][m] + 1)) {
+                rc = GBSW_E_MEMORY;
+                break;
+            }
+            pack(contigs, p0, ro[lane], m, sr.p);
+            pack(reads, p0, qo[lane], m, sq.p);
+            int32_t* o = out.data() + p0;
+            worker[lane] = std::thread([&, lane, ctx, m, o]() {
+                lane_rc[lane] = gbsw_align(ctx, &sc, mode, (int64_t)m, g_stage_ref[lane].p, ro[lane].data(),
+                                           g_stage_qry[lane].p, qo[lane].data(), o, o + n, o + 2 * n, o + 3 * n,
+                                           o + 4 * n);
+                if (lane_rc[lane] != GBSW_OK) lane_err[lane] = gbsw_last_error();
+            });
+        }
+        for (std::thread& w : worker)
+            if (w.joinable()) w.join();
+        for (int lane = 0; lane < kLanes && rc == GBSW_OK; ++lane) {
+            rc = lane_rc[lane];
+            err = lane_err[lane];
         }
     }
     const auto t1 = std::chrono::high_resolution_clock::now();
     g_status = rc;
     if (rc != GBSW_OK) {
-        if (!std::getenv("GBSW_DRIVER_NOEXIT")) die(rc);
+        if (!std::getenv("GBSW_DRIVER_NOEXIT")) die(rc, err);
         return;
     }
     for (size_t k = 0; k < n; ++k) {

@@ -281,6 +281,30 @@ def test_dropin_cpp_caller(tmp_path):
     assert outa[1].split("\t")[0] == "33"      # WWW vs AWWWA: 3 x 11
 
 
+def test_dropin_cpp_caller_large_call_runs_pipelined(tmp_path):
+    """A drop-in call of 131,072 pairs or more runs as sub-batches on two
+    contexts (upload of one beside the alignment of another): its five columns
+    equal the oracle's on every pair."""
+    import subprocess
+    from test_abi import build_dropin_caller
+    exe = build_dropin_caller(tmp_path)
+    w = synth.plain_dna(140_000, seed=0x5EED0301)
+    refs = [w.ref[w.ref_off[k]:w.ref_off[k + 1]].tobytes().decode() for k in range(w.n)]
+    qs = [w.query[w.query_off[k]:w.query_off[k + 1]].tobytes().decode() for k in range(w.n)]
+    fr, fq = tmp_path / "refs.txt", tmp_path / "queries.txt"
+    fr.write_text("\n".join(refs) + "\n")
+    fq.write_text("\n".join(qs) + "\n")
+    print("case dropin pipelined", w.n, flush=True)
+    lines = subprocess.run([exe, "file", str(fr), str(fq)], capture_output=True, text=True, check=True,
+                           timeout=300).stdout.splitlines()
+    rows = np.array([[int(x) for x in l.split("\t")] for l in lines if "\t" in l], dtype=np.int64)
+    assert rows.shape == (w.n, 5)
+    exp = ol.align_batch(w.ref, w.ref_off, w.query, w.query_off, ol.dna_scoring(3, -3, -6, -1))
+    want = np.stack([exp["score"], exp["ref_begin"], exp["ref_end"] - 1, exp["query_begin"], exp["query_end"] - 1], 1)
+    bad = np.nonzero((rows != want).any(1))[0]
+    assert len(bad) == 0, (len(bad), int(bad[0]), rows[bad[0]], want[bad[0]])
+
+
 def test_batched_options_give_identical_results(ctx):
     """Serialised or concurrent buckets, f16 kernels on or off, multi-strip
     pairs split over workgroups or not: the same results (the f16 kernels run
