@@ -184,7 +184,9 @@ typedef struct {
 } gbsw_launch_stat;
 int gbsw_batch_launch_stats(const gbsw_batch* batch, int32_t index, gbsw_launch_stat* out);
 
-/* One call, host memory in and out: upload, align, download (the drop-in path). */
+/* One call, host memory in and out: upload, align, download (the drop-in path).
+ * The context keeps the device buffers of this call's batch for later calls
+ * (they only grow) and frees them in gbsw_context_destroy. */
 int gbsw_align(gbsw_context* ctx, const gbsw_scoring* scoring, int32_t mode, int64_t n,
                const char* ref, const uint64_t* ref_off, const char* query,
                const uint64_t* query_off, int32_t* score, int32_t* ref_begin, int32_t* ref_end,
