@@ -52,6 +52,7 @@ struct gbsw_batch {
     bool aligned = false;  // results of a gbsw_batch_align are in `out`
     // alignment paths: per-pair tables, run buffers, tightness scratch
     DeviceBuffer tb_meta, tb_runs, tb_dir, tb_dst, tb_tables;
+    PinnedBuffer tb_host;
     std::vector<hipEvent_t> tb_events;
     ~gbsw_batch()
     {
@@ -378,14 +379,23 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
         return GBSW_OK;
     }
     hipStream_t st = ctx->ex.stream;
-    std::vector<int32_t> res((size_t)(5 * n));
-    GBSW_HIP_TRY(hipMemcpy(res.data(), b->out.ptr, sizeof(int32_t) * res.size(), hipMemcpyDeviceToHost));
+    // host side in pinned memory, kept with the batch: results (5n int32), then
+    // cap_off (n+1), order (n) and dir_off (n) int64, then run counts (n int32)
+    const size_t hres = sizeof(int32_t) * (size_t)(5 * n), h64 = sizeof(int64_t) * (size_t)(3 * n + 1);
+    GBSW_HIP_TRY(b->tb_host.reserve(hres + h64 + sizeof(int32_t) * (size_t)n));
+    int32_t* res = b->tb_host.as<int32_t>();
+    GBSW_HIP_TRY(hipMemcpyAsync(res, b->out.ptr, hres, hipMemcpyDeviceToHost, st));
+    GBSW_HIP_TRY(hipStreamSynchronize(st));
     // cells are int32: |values| stay below (rows + columns) x the largest |score|
     int32_t big = std::max(std::abs(sc->gap_open), std::abs(sc->gap_extend));
     for (int16_t v : sub) big = std::max<int32_t>(big, std::abs((int32_t)v));
     // per pair: run capacity (rows + columns) and scratch region
     constexpr int64_t kTbScratch = (int64_t)4 << 30;
-    std::vector<int64_t> cap_off((size_t)n + 1), region((size_t)n);
+    int64_t* cap_off = (int64_t*)((uint8_t*)res + hres);
+    int64_t* order = cap_off + n + 1;
+    int64_t* dir_off = order + n;
+    int32_t* nruns = (int32_t*)(dir_off + n);
+    std::vector<int64_t> region((size_t)n);
     std::vector<uint8_t> cls((size_t)n);
     int64_t cls_count[64] = {0};
     cap_off[0] = 0;
@@ -406,7 +416,6 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     }
     // order: longest pairs first (their DP lasts longest; the short ones fill in
     // behind them), by a counting sort over the cost classes
-    std::vector<int64_t> order((size_t)n);
     {
         int64_t start[64];
         int64_t acc = 0;
@@ -414,7 +423,6 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
         for (int64_t p = 0; p < n; ++p) order[(size_t)start[cls[(size_t)p]]++] = p;
     }
     // launches of at most kTbScratch bytes of scratch (or one pair), in that order
-    std::vector<int64_t> dir_off((size_t)n);
     std::vector<std::pair<int64_t, int64_t>> launches; // (first position, count)
     int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
     for (int64_t x = 0; x < n; ++x) {
@@ -439,17 +447,16 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     const size_t half = ((size_t)std::max<int64_t>(8, max_chunk) + 255) & ~(size_t)255;
     GBSW_HIP_TRY(b->tb_dir.reserve(2 * half));
     GBSW_HIP_TRY(b->tb_tables.reserve(256 + sizeof(int16_t) * sub.size()));
+    // same order as the host staging: cap_off, order, dir_off (one copy), then dst_off
     int64_t* d_cap = b->tb_meta.as<int64_t>();
-    int64_t* d_dir_off = d_cap + n + 1;
+    int64_t* d_order = d_cap + n + 1;
+    int64_t* d_dir_off = d_order + n;
     int64_t* d_dst_off = d_dir_off + n;
-    int64_t* d_order = d_dst_off + n + 1;
-    int32_t* d_nruns = (int32_t*)(d_order + n);
+    int32_t* d_nruns = (int32_t*)(d_dst_off + n + 1);
     int32_t* d_score = d_nruns + n;
     uint8_t* d_lut = b->tb_tables.as<uint8_t>();
     int16_t* d_sub = (int16_t*)(d_lut + 256);
-    GBSW_HIP_TRY(hipMemcpyAsync(d_cap, cap_off.data(), sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st));
-    GBSW_HIP_TRY(hipMemcpyAsync(d_dir_off, dir_off.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, st));
-    GBSW_HIP_TRY(hipMemcpyAsync(d_order, order.data(), sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    GBSW_HIP_TRY(hipMemcpyAsync(d_cap, cap_off, sizeof(int64_t) * (size_t)(3 * n + 1), hipMemcpyHostToDevice, st));
     GBSW_HIP_TRY(hipMemcpyAsync(d_lut, b->lut, 256, hipMemcpyHostToDevice, st));
     GBSW_HIP_TRY(hipMemcpyAsync(d_sub, sub.data(), sizeof(int16_t) * sub.size(), hipMemcpyHostToDevice, st));
     TbArgs a{};
@@ -495,12 +502,11 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     }
     GBSW_HIP_TRY(hipStreamWaitEvent(st, b->tb_events[2 * (launches.size() - 1) + 1], 0));
     GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st)); // device_ms: the path kernels
-    std::vector<int32_t> nruns((size_t)n);
-    GBSW_HIP_TRY(hipMemcpyAsync(nruns.data(), d_nruns, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+    GBSW_HIP_TRY(hipMemcpyAsync(nruns, d_nruns, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
     if (path_score)
         GBSW_HIP_TRY(hipMemcpyAsync(path_score, d_score, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
     GBSW_HIP_TRY(hipStreamSynchronize(st));
-    for (int64_t p = 0; p < n; ++p) cigar_offsets[p + 1] = cigar_offsets[p] + nruns[(size_t)p];
+    for (int64_t p = 0; p < n; ++p) cigar_offsets[p + 1] = cigar_offsets[p] + nruns[p];
     const int64_t total = cigar_offsets[n];
     if (device_ms) {
         float ms = 0.f;
@@ -512,7 +518,9 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
                                       " words needed (cigar_offsets holds the layout)");
     }
     GBSW_HIP_TRY(b->tb_dst.reserve(sizeof(uint32_t) * (size_t)std::max<int64_t>(1, total)));
-    GBSW_HIP_TRY(hipMemcpyAsync(d_dst_off, cigar_offsets, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st));
+    // the offsets through pinned memory (the order and dir_off staging is free now)
+    std::memcpy(order, cigar_offsets, sizeof(int64_t) * (size_t)(n + 1));
+    GBSW_HIP_TRY(hipMemcpyAsync(d_dst_off, order, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st));
     GBSW_HIP_TRY(tb_compact(a.runs, d_cap, d_nruns, d_dst_off, b->tb_dst.as<uint32_t>(), n, st));
     if (total)
         GBSW_HIP_TRY(hipMemcpyAsync(cigar, b->tb_dst.ptr, sizeof(uint32_t) * (size_t)total, hipMemcpyDeviceToHost, st));
