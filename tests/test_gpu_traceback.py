@@ -87,6 +87,26 @@ def test_long_and_uneven_dna_paths(ctx):
     assert cols.max() > 640
 
 
+def test_long_rectangles(ctx):
+    # rectangles of up to ~4,000 x 4,000 cells: tens of stripes, many ring refills
+    rng = np.random.default_rng(31)
+    pairs = []
+    for k in range(24):
+        n = int(rng.integers(1500, 4000))
+        ref = rng.choice(np.frombuffer(b"ACGT", np.uint8), n)
+        q = ref[int(rng.integers(0, 200)):].copy()
+        mut = rng.random(len(q)) < 0.08
+        q[mut] = rng.choice(np.frombuffer(b"ACGT", np.uint8), int(mut.sum()))
+        if k % 3 == 0:  # a few indels
+            cut = int(rng.integers(10, len(q) - 10))
+            q = np.concatenate([q[:cut], q[cut + 3:]])
+        pairs.append((ref.tobytes(), q.tobytes()))
+    rb, ro, qb, qo = pack(pairs)
+    words, off, ps, res = check_paths(ctx, gbsw.DNA, rb, ro, qb, qo, gbsw.dna_scoring(3, -3, -6, -1),
+                                      ol.dna_scoring(3, -3, -6, -1), "long rectangles")
+    assert (res["query_end"] - res["query_begin"]).max() > 1400
+
+
 def test_edge_cases(ctx):
     rng = random.Random(9)
     pairs = [
