@@ -263,8 +263,9 @@ def main():
                     help="also run one device-to-device copy of this size (rocprof FETCH/WRITE calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (pairs are independent; "
-              "sharding by cell count is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here (pairs are independent; "
+              "several GPUs would each take a share of the pairs balanced by cell count). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     t = time.perf_counter()

@@ -43,7 +43,7 @@ def rand_seq(rng, n, alpha):
 # ---------------------------------------------------------------- fixtures
 
 
-def test_survey_tie_fixture():
+def test_full_tie_prefers_the_lower_tree_priority():
     """Full tie in row 9 of the forward pass: column t=8 has the smaller
     bit-reversed tree priority than t=3 (kernel.cpp:59-80)."""
     sc = ol.dna_scoring(1, -3, -3, -1)
