@@ -131,6 +131,41 @@ def traceback_bench(batch, scoring, res, w, args):
     return out
 
 
+def dropin_bench(w, res, args):
+    """End-to-end calls of the C++ drop-in (gpu_bsw_driver::kernel_driver_dna) on the
+    workload, timed the way the reference reports it: its driver's "Total Execution
+    Time" line (packing, transfers, both passes, copy back; driver.cpp:34, 240-242).
+    The sums of scores and coordinates of the last call must equal the batch's."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(REPO, "gpu-bsw_amd", "dropin_time")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(REPO, "gpu-bsw_amd"), "dropin_time"], check=True)
+    tmp = tempfile.mkdtemp(prefix="gbsw_dropin_")
+    try:
+        fr, fq = os.path.join(tmp, "refs.txt"), os.path.join(tmp, "queries.txt")
+        for path, buf, off in ((fr, w.ref, w.ref_off), (fq, w.query, w.query_off)):
+            # one sequence per line: the bytes with a newline after each pair's end
+            lines = np.insert(np.asarray(buf, dtype=np.uint8), np.asarray(off[1:], dtype=np.int64), ord("\n"))
+            lines.tofile(path)
+        out = subprocess.run([exe, fr, fq, str(args.dropin_reps + 1)], capture_output=True, text=True, check=True,
+                             timeout=600).stdout.splitlines()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    times = [float(l.split(":")[1]) for l in out if l.startswith("Total Execution Time (seconds):")]
+    ck = [l.split() for l in out if l.startswith("checksum")][0]
+    want_score = int(np.sum(res["score"], dtype=np.int64))
+    want_coord = int(np.sum(res["ref_begin"], dtype=np.int64) + np.sum(res["ref_end"], dtype=np.int64) +
+                     np.sum(res["query_begin"], dtype=np.int64) + np.sum(res["query_end"], dtype=np.int64))
+    same = int(ck[1]) == want_score and int(ck[2]) == want_coord
+    t = min(times[1:]) if len(times) > 1 else times[0]
+    log("drop-in end to end: %d pairs, Total Execution Time %.4f s (best of %d after one warm-up call), "
+        "%.1f GCUPS; sums equal to the batch's: %s" % (w.n, t, len(times) - 1, w.cells() / t / 1e9, same))
+    return {"pairs": int(w.n), "total_execution_time_s": round(t, 4), "gcups": round(w.cells() / t / 1e9, 2),
+            "calls": len(times), "sums_equal_batch": bool(same)}
+
+
 def time_mode(batch, scoring, mode, warmup, steps, cells):
     for _ in range(warmup):
         batch.align(scoring, mode)
@@ -253,6 +288,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default length -> geometry map")
     ap.add_argument("--long-pairs", type=int, default=50_000, help="pairs of the long-protein set")
     ap.add_argument("--no-traceback", action="store_true", help="skip the alignment-path (CIGAR) line")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the end-to-end drop-in calls")
+    ap.add_argument("--dropin-reps", type=int, default=3)
     ap.add_argument("--traceback-steps", type=int, default=3)
     ap.add_argument("--traceback-cpu-pairs", type=int, default=50_000)
     ap.add_argument("--serving-window", type=float, default=5.0, help="seconds per offered load")
@@ -364,6 +401,12 @@ def main():
 
     batch.close()
     ctx.close()
+    dropin = None
+    if not args.no_dropin:
+        dropin = dropin_bench(w, res_b, args)
+        if not dropin["sums_equal_batch"]:
+            log("ERROR: the drop-in's results differ from the batch's")
+            sys.exit(1)
     configs = None
     if not args.no_configs:
         configs = config_bench(args)
@@ -410,6 +453,7 @@ def main():
         "configs": configs,
         "serving": serving,
         "traceback": tb,
+        "dropin_end_to_end": dropin,
     }
     print(json.dumps(line), flush=True)
 
