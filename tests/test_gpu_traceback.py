@@ -107,6 +107,14 @@ def test_long_rectangles(ctx):
     assert (res["query_end"] - res["query_begin"]).max() > 1400
 
 
+def test_long_protein_paths(ctx):
+    # the long-protein generator's lengths (queries 1,024-4,096, refs 4,096-16,384)
+    # on a few pairs: int8-profile alignments, rectangles of thousands of columns
+    w = synth.long_protein(12, seed=0x5EED0401)
+    check_paths(ctx, gbsw.PROTEIN, w.ref, w.ref_off, w.query, w.query_off, gbsw.protein_scoring(BLOSUM62, *w.scoring),
+                ol.aa_scoring(BLOSUM62, *w.scoring), "long_protein 12")
+
+
 def test_edge_cases(ctx):
     rng = random.Random(9)
     pairs = [
