@@ -478,9 +478,11 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     a.path_score = d_score;
     a.order = d_order;
     Exec& ex = ctx->ex;
+    // DP kernels on the context stream, walks on an auxiliary one (created here
+    // when the context runs its buckets on one stream); launch c's walk waits for
+    // its DP, and launch c + 2's DP (same scratch half) for that walk
+    if (int rc = ex.ensure_streams(2)) return rc;
     GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st));
-    // DP kernels on the context stream, walks on an auxiliary one; launch c's walk
-    // waits for its DP, and launch c + 2's DP (same scratch half) for that walk
     hipStream_t sw = ex.aux[0];
     while (b->tb_events.size() < 2 * launches.size()) {
         hipEvent_t e;
@@ -580,6 +582,7 @@ int gbsw_batch_autotune(gbsw_batch* b, const gbsw_scoring* sc, int32_t repeats, 
         GBSW_HIP_TRY(hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, m).data(), 65536, hipMemcpyHostToDevice));
         job.kmodel = m;
         count_buckets(job, b->len_ref.data(), b->len_qry.data());
+        b->graph.reset(); // its launches hold the previous map's bucket slices
         return GBSW_OK;
     };
     // the batch's current model first; another one must beat it by more than

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -116,8 +117,28 @@ void pack(const std::vector<std::string>& v, size_t first, const std::vector<uin
     std::exit(1);
 }
 
+// Sub-batches of a call: four for large calls (pipelining), and at least as
+// many as keep the device footprint of the two sub-batches in flight within
+// factor x device memory.  As in the reference (driver.cpp:36, 51-55) a pair is
+// priced at the longest ref plus the longest query; here plus kPairBytes of
+// offsets, results and binning arrays per pair.
+size_t sub_batches(size_t n, size_t max_ref, size_t max_qry, float factor)
+{
+    constexpr size_t kPairBytes = 96;
+    size_t S = n >= kPipelinePairs ? kSubBatches : 1;
+    int dev = 0;
+    if (const char* e = std::getenv("GBSW_DEVICE")) dev = std::atoi(e);
+    else (void)hipGetDevice(&dev);
+    uint64_t mem = 0;
+    if (n == 0 || !(factor > 0.f) || gbsw_device_memory(dev, &mem) != GBSW_OK) return S;
+    const double budget = (double)mem * (double)factor / kLanes;
+    const double per_pair = (double)(max_ref + max_qry + kPairBytes);
+    const size_t fit = (size_t)std::max(1.0, budget / per_pair);
+    return std::max(S, (n + fit - 1) / fit);
+}
+
 void run(std::vector<std::string>& reads, std::vector<std::string>& contigs,
-         gpu_bsw_driver::alignment_results* alignments, const gbsw_scoring& sc)
+         gpu_bsw_driver::alignment_results* alignments, const gbsw_scoring& sc, float factor)
 {
     const size_t n = contigs.size(); // driver.cpp:21 pairs reads[k] with contigs[k]
     g_status = GBSW_OK;
@@ -147,19 +168,19 @@ void run(std::vector<std::string>& reads, std::vector<std::string>& contigs,
         std::lock_guard<std::mutex> lk(g_stage_mu);
         // sub-batch k: pairs [n k / S, n (k+1) / S) on lane k % 2; the lane's
         // previous sub-batch has finished before its staging is refilled
-        const int S = n >= kPipelinePairs ? kSubBatches : 1;
+        const size_t S = sub_batches(n, max_ref, max_qry, factor);
         std::vector<uint64_t> ro[kLanes], qo[kLanes];
         std::thread worker[kLanes];
         int lane_rc[kLanes] = {GBSW_OK, GBSW_OK};
         std::string lane_err[kLanes];
         const int32_t mode = driver_mode();
-        for (int k = 0; k < S && rc == GBSW_OK; ++k) {
-            const int lane = k % kLanes;
+        for (size_t k = 0; k < S && rc == GBSW_OK; ++k) {
+            const int lane = (int)(k % kLanes);
             if (worker[lane].joinable()) worker[lane].join();
             if (lane_rc[lane] != GBSW_OK) break;
             gbsw_context* ctx = shared_context(lane, &rc);
             if (rc != GBSW_OK) break;
-            const size_t p0 = n * (size_t)k / (size_t)S, m = n * (size_t)(k + 1) / (size_t)S - p0;
+            const size_t p0 = n * k / S, m = n * (k + 1) / S - p0;
             offsets(contigs, p0, ro[lane], m);
             offsets(reads, p0, qo[lane], m);
             Staging& sr = g_stage_ref[lane];
@@ -191,12 +212,30 @@ void run(std::vector<std::string>& reads, std::vector<std::string>& contigs,
         if (!std::getenv("GBSW_DRIVER_NOEXIT")) die(rc, err);
         return;
     }
+    // The contract's arrays are `short` (driver.hpp:21-27), whose values the
+    // reference would wrap past 32767; here such a call reports GBSW_E_RANGE
+    // (exit unless GBSW_DRIVER_NOEXIT) and the arrays hold the values clamped to
+    // the short range.  gbsw_align returns the int32 values.
+    size_t wide = 0;
+    auto to_short = [&](int32_t v) -> short {
+        if (v > SHRT_MAX || v < SHRT_MIN) {
+            ++wide;
+            return (short)(v > SHRT_MAX ? SHRT_MAX : SHRT_MIN);
+        }
+        return (short)v;
+    };
     for (size_t k = 0; k < n; ++k) {
-        alignments->top_scores[k] = (short)out[k];
-        alignments->ref_begin[k] = (short)out[n + k];
-        alignments->ref_end[k] = (short)out[2 * n + k];
-        alignments->query_begin[k] = (short)out[3 * n + k];
-        alignments->query_end[k] = (short)out[4 * n + k];
+        alignments->top_scores[k] = to_short(out[k]);
+        alignments->ref_begin[k] = to_short(out[n + k]);
+        alignments->ref_end[k] = to_short(out[2 * n + k]);
+        alignments->query_begin[k] = to_short(out[3 * n + k]);
+        alignments->query_end[k] = to_short(out[4 * n + k]);
+    }
+    if (wide) {
+        g_status = GBSW_E_RANGE;
+        const std::string msg = std::to_string(wide) + " result values exceed the short range of alignment_results";
+        if (!std::getenv("GBSW_DRIVER_NOEXIT")) die(g_status, msg);
+        std::fprintf(stderr, "gpu_bsw_driver: %s (status %d)\n", msg.c_str(), g_status);
     }
     const std::chrono::duration<double> dt = t1 - t0;
     std::cout << "Total Alignments:" << n << "\n"
@@ -219,7 +258,6 @@ size_t get_tot_gpu_mem(int id)
 void kernel_driver_dna(std::vector<std::string> reads, std::vector<std::string> contigs,
                        alignment_results* alignments, short scores[4], float factor)
 {
-    (void)factor;
     gbsw_scoring sc;
     std::memset(&sc, 0, sizeof(sc));
     sc.alphabet = GBSW_DNA;
@@ -227,21 +265,20 @@ void kernel_driver_dna(std::vector<std::string> reads, std::vector<std::string> 
     sc.mismatch = scores[1];
     sc.gap_open = scores[2];
     sc.gap_extend = scores[3];
-    run(reads, contigs, alignments, sc);
+    run(reads, contigs, alignments, sc, factor);
 }
 
 void kernel_driver_aa(std::vector<std::string> reads, std::vector<std::string> contigs,
                       alignment_results* alignments, short scoring_matrix[], short openGap, short extendGap,
                       float factor)
 {
-    (void)factor;
     gbsw_scoring sc;
     std::memset(&sc, 0, sizeof(sc));
     sc.alphabet = GBSW_PROTEIN;
     sc.gap_open = openGap;
     sc.gap_extend = extendGap;
     std::memcpy(sc.matrix, scoring_matrix, sizeof(sc.matrix)); // SCORE_MAT_SIZE = 576 (kernel.hpp:9)
-    run(reads, contigs, alignments, sc);
+    run(reads, contigs, alignments, sc, factor);
 }
 
 int last_status() { return g_status; }

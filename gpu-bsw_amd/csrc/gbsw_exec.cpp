@@ -441,6 +441,15 @@ bool plan_is_split(const BucketPlan& b)
     return k == kSplit || k == kSplitF16;
 }
 
+// 32-bit words of a split bucket's bookkeeping: keys (2 x count u64), progress
+// words (2 slots per workgroup, u64), done counters (2 x quads), release epochs
+// (one per slot) and the placement table (kXcds).
+size_t split_words_of(const Bucket& bk, const BucketPlan& b)
+{
+    const size_t nquads = (size_t)((bk.count + kPairsPerWave - 1) / kPairsPerWave);
+    return 4 * (size_t)bk.count + 2 * nquads + 3 * (size_t)(2 * b.grid) + kXcds;
+}
+
 // Kernels, grids and scratch of every bucket (host only; reserves the buffers).
 int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int64_t bnd_rows,
                  std::vector<BucketPlan>& plan, std::vector<size_t>& order)
@@ -463,7 +472,7 @@ int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int
         // workgroup busy for two rounds (on the 200k-pair DNA mix: 1765 vs 827
         // GCUPS at 1.4 quads per workgroup; 988 vs 1275 on protein at 2.7).
         const int64_t full = (int64_t)std::max(1, kernel_occupancy(b.kind, b.K, b.lds)) * ex.num_cu;
-        if (ex.split && b.strips > 1 && nquads < 2 * full && kernel_has(skind, b.K)) {
+        if (ex.split && !ex.split_misplaced && b.strips > 1 && nquads < 2 * full && kernel_has(skind, b.K)) {
             // every XCD's counter needs workgroups: a multiple of kXcds, no more
             // than the items a pipelined quad can keep busy, and two boundary
             // slots per workgroup
@@ -487,8 +496,7 @@ int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int
             if (plan_is_split(b)) {
                 bnd_words += (size_t)2 * b.grid * kPairsPerWave * (size_t)bnd_rows;
                 b.split_at = split_words;
-                split_words += 4 * (size_t)bk.count + 2 * (size_t)((bk.count + kPairsPerWave - 1) / kPairsPerWave) +
-                               3 * (size_t)(2 * b.grid);
+                split_words += split_words_of(bk, b);
             } else {
                 bnd_words += (size_t)b.grid * kPairsPerWave * (size_t)bnd_rows;
             }
@@ -544,9 +552,7 @@ int record_batched(Exec& ex, const Job& job, KernelArgs a, int max_sub, const st
     size_t split_words = 0;
     for (size_t i = 0; i < nb; ++i)
         if (plan_is_split(plan[i]))
-            split_words = std::max(split_words, plan[i].split_at + 4 * (size_t)job.buckets[i].count +
-                                                    2 * (size_t)((job.buckets[i].count + kPairsPerWave - 1) / kPairsPerWave) +
-                                                    3 * (size_t)(2 * plan[i].grid));
+            split_words = std::max(split_words, plan[i].split_at + split_words_of(job.buckets[i], plan[i]));
     if (split_words) GBSW_HIP_TRY(zero_u32(ex.split_buf.as<uint32_t>(), (int)split_words, st0));
     const int ns = std::max(1, std::min(kAuxStreams + 1, ex.streams));
     GBSW_HIP_TRY(hipEventRecord(ex.ev_fork, st0));
@@ -577,6 +583,7 @@ int record_batched(Exec& ex, const Job& job, KernelArgs a, int max_sub, const st
             a.split.prog = (uint64_t*)(w + 4 * bk.count);
             a.split.done = w + 4 * bk.count + 2 * nslots;
             a.split.rel = a.split.done + 2 * nquads;
+            a.split.xcc = a.split.rel + nslots;
             a.split.strips = b.strips;
             a.split.slots = (int32_t)(nslots / kXcds);
         }
@@ -595,9 +602,12 @@ int record_batched(Exec& ex, const Job& job, KernelArgs a, int max_sub, const st
     return GBSW_OK;
 }
 
-// What a captured graph depends on besides the batch itself: the scoring, the
-// options, and the addresses of the context buffers it writes.
-std::string graph_key(const Exec& ex, const gbsw_scoring* sc, const std::vector<BucketPlan>& plan)
+// What a captured graph depends on besides the batch's bytes: the scoring, the
+// options, the addresses of the context buffers it writes, and each bucket's
+// slice of the sort order (baked into the captured launch arguments; another
+// length -> geometry map of the same batch moves the slices while the plans may
+// stay alike).
+std::string graph_key(const Exec& ex, const Job& job, const gbsw_scoring* sc, const std::vector<BucketPlan>& plan)
 {
     // DNA reads the first four scores only
     const size_t sc_bytes = sc->alphabet == GBSW_DNA ? offsetof(gbsw_scoring, matrix) : sizeof(*sc);
@@ -606,10 +616,15 @@ std::string graph_key(const Exec& ex, const gbsw_scoring* sc, const std::vector<
                           ex.sub.ptr,   ex.lut.ptr,            ex.colmax.ptr,    ex.tables_host.ptr, ex.status_host.ptr,
                           ex.stream};
     k.append((const char*)ptrs, sizeof(ptrs));
-    const int opts[] = {ex.streams, ex.f16 ? 1 : 0, ex.split ? 1 : 0, (int)plan.size()};
+    const int opts[] = {ex.streams, ex.f16 ? 1 : 0, (ex.split && !ex.split_misplaced) ? 1 : 0, (int)plan.size()};
     k.append((const char*)opts, sizeof(opts));
-    for (const BucketPlan& b : plan) {
-        const int64_t f[] = {b.kind, b.K, b.grid, b.lds, b.strips, (int64_t)b.bnd_at, (int64_t)b.split_at};
+    const int64_t jf[] = {job.kmodel, job.n, (int64_t)(uintptr_t)job.order};
+    k.append((const char*)jf, sizeof(jf));
+    for (size_t i = 0; i < plan.size(); ++i) {
+        const BucketPlan& b = plan[i];
+        const Bucket& bk = job.buckets[i];
+        const int64_t f[] = {b.kind, b.K, b.grid, b.lds, b.strips, (int64_t)b.bnd_at, (int64_t)b.split_at,
+                             bk.begin, bk.count, bk.max_cols};
         k.append((const char*)f, sizeof(f));
     }
     return k;
@@ -698,7 +713,7 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     // once and replayed (one launch instead of ~20 API calls); the per-launch
     // records of the capture stand for every replay
     JobGraph& G = *job.graph;
-    const std::string key = graph_key(ex, sc, plan);
+    const std::string key = graph_key(ex, job, sc, plan);
     if (!G.exec || G.key != key) {
         G.reset();
         GBSW_HIP_TRY(hipStreamBeginCapture(st0, hipStreamCaptureModeRelaxed));
@@ -734,7 +749,19 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
 int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* device_ms)
 {
     GBSW_HIP_TRY(hipEventSynchronize(ex.ev_stop));
+    ex.late_results = false;
     int32_t* status = ex.status_host.as<int32_t>();
+    if (split_rerun_needed(status) && !ex.split_misplaced) {
+        // a split launch found workgroups of one XCD counter on different XCCs,
+        // so its boundary rows may have been read stale: the whole job again,
+        // with the split form off from now on (it is only used in the batched form)
+        ex.split_misplaced = true;
+        set_error("split form disabled: workgroups of one XCD counter ran on different XCCs");
+        int rc = enqueue_job(ex, job, sc, GBSW_BATCHED);
+        if (!rc) rc = complete_job(ex, job, sc, device_ms);
+        ex.late_results = true;
+        return rc;
+    }
     if (status[kStatusSaturated] > 0) {
         // pairs whose int16 score reached the saturation limit: recompute them
         // (forward + reverse) with int32 cells
@@ -761,6 +788,7 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
                                     hipMemcpyDeviceToHost, ex.stream));
         GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, ex.stream));
         GBSW_HIP_TRY(hipEventSynchronize(ex.ev_stop));
+        ex.late_results = true;
     }
     float ms = 0.f;
     GBSW_HIP_TRY(hipEventElapsedTime(&ms, ex.ev_start, ex.ev_stop));
@@ -774,6 +802,7 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
         }
         ex.launches[i].ms = t;
     }
+    if (status[kStatusMisplaced]) return fail(GBSW_E_INTERNAL, "split-form placement check failed after an unsplit rerun");
     if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
     if (status[kStatusTimeout]) return fail(GBSW_E_INTERNAL, "a split-form wait timed out");
     return GBSW_OK;

@@ -84,6 +84,10 @@ struct Exec {
     int ensure_streams(int n);
     bool f16 = true;               // exact f16 kernels where the score bound allows
     bool split = true;             // multi-strip buckets run their strips on several workgroups
+    bool split_misplaced = false;  // a split launch ran workgroups of one XCD counter on different
+                                   // XCCs: the split form stays off for this Exec
+    bool late_results = false;     // last complete_job wrote results after the job's own copies
+                                   // (int32 recompute or unsplit rerun)
     bool fixed = false;            // buffers preallocated (serving slot): never grown by a job
     bool graphs = true;            // batched form of a resident batch replays a captured hipGraph
     int graph_state = 0;           // last job: 0 eager, 1 captured, 2 replayed

@@ -2,6 +2,12 @@
 #pragma once
 #include <stdint.h>
 
+// The pure helpers below also build in host-only C++ (CPU tests).
+#if !defined(__HIPCC__) && !defined(__host__)
+#define __host__
+#define __device__
+#endif
+
 namespace gbsw {
 
 constexpr int kLanesPerPair = 16;              // one DPP row
@@ -28,7 +34,14 @@ constexpr int16_t kNeg = -16384;
 enum { kOutScore = 0, kOutRefBegin = 1, kOutRefEnd = 2, kOutQueryBegin = 3, kOutQueryEnd = 4 };
 
 // status[] words written by kernels
-enum { kStatusBadResidue = 0, kStatusSaturated = 1, kStatusRevMiss = 2, kStatusTimeout = 3, kStatusWords = 4 };
+enum {
+    kStatusBadResidue = 0,
+    kStatusSaturated = 1,
+    kStatusRevMiss = 2,
+    kStatusTimeout = 3,
+    kStatusMisplaced = 4, // k_split: workgroups of one XCD counter ran on different XCCs
+    kStatusWords = 5
+};
 
 // Split form (k_split): the strips of a multi-strip pair run on different
 // workgroups of one XCD, each strip following its predecessor's boundary rows.
@@ -37,11 +50,34 @@ struct SplitArgs {
     uint32_t* done;   // 2 x quads: forward, then reverse strip items finished
     uint64_t* prog;   // per boundary slot: epoch << 32 | boundary rows ready
     uint32_t* rel;    // per boundary slot: epochs released
+    uint32_t* xcc;    // kXcds words: 1 + the hardware XCC id seen by counter x's workgroups (0: none yet)
     int32_t strips;   // strip items per quad and pass
     int32_t slots;    // boundary slots per XCD
 };
-constexpr int kXcds = 8;                 // workgroup w runs on XCD w % 8
+constexpr int kXcds = 8;                 // workgroup w is expected on XCD w % 8
 constexpr uint64_t kSpinTicks = 200000000ull; // 2 s of the 100 MHz s_memrealtime clock
+
+// Placement check of the split form.  Its producer and consumer strips exchange
+// boundary rows through plain stores, which are visible to each other only
+// inside one XCC's L2, so every workgroup serving counter x must run on one XCC.
+// Each workgroup compare-and-swaps (1 + its XCC id) into slot x of a zeroed
+// table; `prev` is what the slot held before.  The placement is consistent when
+// the slot was empty or already held this XCC.  Whatever the hardware's
+// workgroup -> XCC mapping (w % 8 on an 8-XCC partition, fewer XCCs per
+// partition in other modes), only a mapping that puts two workgroups of one
+// counter on different XCCs fails, and then the host reruns the job unsplit.
+__host__ __device__ constexpr bool placement_consistent(uint32_t prev, uint32_t xcc)
+{
+    return prev == 0u || prev == xcc + 1u;
+}
+// The host's decision after a job: rerun it with the split form off.
+__host__ __device__ constexpr bool split_rerun_needed(const int32_t* status)
+{
+    return status[kStatusMisplaced] != 0;
+}
+// s_getreg_b32 operand of HW_REG_XCC_ID (register 20, bits 3:0): the XCC the
+// wave runs on (gfx942 / gfx950).
+constexpr int kXccIdReg = (3 << 11) | (0 << 6) | 20;
 
 struct KernelArgs {
     const uint8_t* ref;

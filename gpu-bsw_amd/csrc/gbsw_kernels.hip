@@ -1074,9 +1074,11 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
 // Split form of k_batched for buckets of multi-strip pairs: a quad's strips run
 // on different workgroups, pipelined, instead of one after another on one.
 //
-// Workgroup w serves XCD x = w % 8 (where the hardware places it), with its own
-// work counter a.queue[x] and boundary slots, so producer and consumer of a
-// boundary share an L2.  XCD x owns quads x, x + 8, ...; its items are first
+// Workgroup w serves XCD x = w % 8 (where the hardware places it on an 8-XCC
+// partition), with its own work counter a.queue[x] and boundary slots, so
+// producer and consumer of a boundary share an L2.  Each workgroup checks that
+// assumption at entry against the hardware's XCC id (placement_consistent); a
+// violation sets kStatusMisplaced and the host reruns the job unsplit.  XCD x owns quads x, x + 8, ...; its items are first
 // (quad, strip) of the forward pass for all its quads, then the same for the
 // reverse pass (a.split.strips items per quad and pass; strips past a quad's
 // count are empty items).  Item i writes its outgoing boundary into slot
@@ -1100,6 +1102,17 @@ __global__ __launch_bounds__(64) void k_split(KernelArgs a)
     const int g = threadIdx.x >> 4;
     const int x = (int)(blockIdx.x % kXcds);
     const SplitArgs& sp = a.split;
+    // every workgroup of counter x must share one XCC (its L2): record this
+    // one's, and flag the job for an unsplit rerun when they differ
+    if (threadIdx.x == 0) {
+        const uint32_t xcc = (uint32_t)__builtin_amdgcn_s_getreg(kXccIdReg) & 15u;
+        const uint32_t prev = atomicCAS(sp.xcc + x, 0u, xcc + 1u);
+        if (!placement_consistent(prev, xcc)) {
+            // the results of this job are void: let every wait return at once
+            atomicOr(a.status + kStatusMisplaced, 1);
+            atomicOr(a.status + kStatusTimeout, 1);
+        }
+    }
     const int64_t nquads = (a.order_n + kPairsPerWave - 1) / kPairsPerWave;
     const int64_t nq_x = nquads > x ? (nquads - x + kXcds - 1) / kXcds : 0;
     const int64_t per_pass = nq_x * sp.strips;

@@ -319,7 +319,8 @@ int gbsw_server::init_slot(Slot& s)
     if (!rc) rc = alloc_into(s.ex.queue, 4 * kXcds * kMaxBuckets, device_bytes);
     // split-form bookkeeping: keys and counters of every pair, and the progress
     // words of up to two boundary slots per workgroup of a full grid per bucket
-    const size_t split_words = 4 * P + 2 * (P / kPairsPerWave + kMaxBuckets) + 6 * (size_t)kMaxBuckets * 16 * s.ex.num_cu;
+    const size_t split_words = 4 * P + 2 * (P / kPairsPerWave + kMaxBuckets) +
+                               6 * (size_t)kMaxBuckets * 16 * s.ex.num_cu + (size_t)kXcds * kMaxBuckets;
     if (!rc) rc = alloc_into(s.ex.split_buf, 4 * split_words, device_bytes);
     s.ex.fixed = true;
     if (rc) return rc;
@@ -562,10 +563,9 @@ void gbsw_server::finish(Slot& s, bool device_ok)
     if (rc == GBSW_OK && device_ok) {
         double ms = 0.0;
         rc = complete_job(s.ex, s.job, &s.sc, &ms);
-        bool redone = false;
-        for (const LaunchRec& L : s.ex.launches) redone |= (L.kind & ~kByteProfile) == kBatchedWide;
-        if (rc == GBSW_OK && redone) {
-            // int32 recompute ran after the copy-back: copy the results again
+        if (rc == GBSW_OK && s.ex.late_results) {
+            // an int32 recompute or an unsplit rerun ran after the copy-back:
+            // copy the results again
             hipError_t e = hipMemcpyAsync(s.out_host.ptr, s.out.ptr, sizeof(int32_t) * 5 * (size_t)s.job.n,
                                           hipMemcpyDeviceToHost, s.ex.stream);
             if (e == hipSuccess) e = hipStreamSynchronize(s.ex.stream);

@@ -15,10 +15,13 @@
 // them with free_alignments; ends are 1-based, begins 0-based; errors print a
 // message and exit the process (utils.hpp:12-21), unless GBSW_DRIVER_NOEXIT is set
 // in the environment, in which case the results are zero-filled and
-// gpu_bsw_driver::last_status() reports the error.
+// gpu_bsw_driver::last_status() reports the error; `factor` bounds the device
+// memory a call uses to factor x the device's memory (driver.cpp:36, 51-55): the
+// pairs run in as many sub-batches as that takes.
 // Behaviour changed: one device (the current HIP device, or GBSW_DEVICE), no
-// OpenMP side effects, the `factor` memory fraction is accepted and ignored
-// (batches are resident in HBM as a whole).
+// OpenMP side effects; a result value outside the `short` range (scores or
+// coordinates above 32767, which the reference wraps) is an error,
+// GBSW_E_RANGE, with the arrays holding the values clamped to that range.
 #ifndef GPU_BSW_DRIVER_HPP
 #define GPU_BSW_DRIVER_HPP
 

@@ -6,16 +6,16 @@ Run in the build container (where the read-only reference checkout exists):
 Only data files are read (no reference code is imported or run):
   test-data/protein-query.fasta                         (30,000 single-line records)
   regression_test/test_regression_protein_good_result   (30,000 golden top scores)
-  regression_test/test_regression_dna_good_result       (30,000 golden top scores)
 
 Output:
   protein_query_golden.tsv : first N records as "<query>\t<golden score>"
-  dna_golden_scores.txt    : the 30,000 DNA golden scores (inputs are not shipped)
   reference_tables.json    : the numbers of the reference's residue encoding
                              (driver.cpp:252-258) and of the BLOSUM62 matrix its
                              sample program passes (main.cpp:70-92), parsed as data
 The protein reference sequences and all DNA inputs are missing from the
-reference checkout (.MISSING_LARGE_BLOBS), so the goldens can only bound scores.
+reference checkout (.MISSING_LARGE_BLOBS), so the protein goldens can only bound
+scores, and the DNA goldens (regression_test/test_regression_dna_good_result),
+whose inputs are all missing, pin nothing and are not kept.
 """
 import json
 import os
@@ -47,10 +47,6 @@ def main(ref_root):
     with open(os.path.join(here, "protein_query_golden.tsv"), "w") as out:
         for q, g in zip(queries[:N_PROTEIN], golden[:N_PROTEIN]):
             out.write("%s\t%d\n" % (q, g))
-    with open(os.path.join(ref_root, "regression_test", "test_regression_dna_good_result")) as fh:
-        dna = [int(x.split()[0]) for x in fh if x.strip()]
-    with open(os.path.join(here, "dna_golden_scores.txt"), "w") as out:
-        out.write("\n".join(str(v) for v in dna) + "\n")
     drv = open(os.path.join(ref_root, "driver.cpp")).read()
     a = drv.index("short encoding_matrix[] = {")
     enc = [int(x) for x in re.findall(r"-?\d+", drv[drv.index("{", a) + 1:drv.index("}", a)])]
@@ -61,7 +57,7 @@ def main(ref_root):
     with open(os.path.join(here, "reference_tables.json"), "w") as out:
         json.dump({"source": "numbers parsed from driver.cpp:252-258 (encoding_matrix) and main.cpp:70-92 "
                              "(scores_matrix) of the reference", "encoding": enc, "blosum62": blosum}, out)
-    print("wrote %d protein and %d DNA golden rows, reference tables" % (N_PROTEIN, len(dna)))
+    print("wrote %d protein golden rows and the reference tables" % N_PROTEIN)
 
 
 if __name__ == "__main__":
