@@ -420,3 +420,27 @@ def test_protein_scores_beyond_int8_use_the_int16_profile(ctx):
         b.align(gbsw.protein_scoring(BLOSUM62, -6, -1), mode)
         assert all(L["kind"] >= gbsw.KIND_BYTE_PROFILE for L in b.launches())
         b.close()
+
+
+def test_autotune_then_replay_uses_the_kept_map(ctx):
+    """A batch large enough for its buckets to fill the grid (several maps give
+    the same launch plans): after gbsw_batch_autotune has tried the maps, the
+    results are overwritten by a plain run of another scoring, and the next
+    batched align (a graph replay or a new capture) must use the kept map's
+    bucket slices and give the oracle's results."""
+    w = synth.batched_dna(200_000, seed=0x5EED020B)
+    exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, gbsw.DNA, w.scoring)
+    other = oracle_results(w.ref, w.ref_off, w.query, w.query_off, gbsw.DNA, (1, -3, -3, -1))
+    sc = gbsw.dna_scoring(*w.scoring)
+    b = ctx.batch(gbsw.DNA, w.ref, w.ref_off, w.query, w.query_off)
+    try:
+        print("case autotune then replay", flush=True)
+        b.autotune(sc, repeats=1)
+        for rep in range(2):
+            b.align(gbsw.dna_scoring(1, -3, -3, -1), "plain")
+            assert_same(b.results(), other, "overwrite %d" % rep)
+            b.align(sc, "batched")
+            assert_same(b.results(), exp, "after autotune %d" % rep)
+    finally:
+        b.close()
+        ctx.set_option(gbsw.OPT_GEOMETRY, 1)

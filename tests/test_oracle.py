@@ -427,3 +427,72 @@ def test_tables_match_reference_data():
     assert [encode_aa(b) for b in range(91)] == ref["encoding"]
     assert BLOSUM62 == ref["blosum62"]
     assert ol.encode_aa(91) == -1 and ol.encode_aa(-1) == -1
+
+
+# ------------------------------------------------ asymmetric substitution tables
+
+
+def _asym_table(rng, filler):
+    """A random 24x24 table with M[a][b] != M[b][a] for most pairs (the reference
+    takes any 576 entries, driver.cpp:304-308); residues of `filler` score
+    negatively against every other residue."""
+    fill = {encode_aa(ord(x)) for x in filler}
+    m = []
+    for r in range(24):
+        for c in range(24):
+            if r == c:
+                m.append(rng.randint(4, 9))
+            elif r in fill or c in fill:
+                m.append(rng.randint(-6, -2))
+            else:
+                m.append(rng.randint(-6, 3))
+    return m
+
+
+def _asym_cases(seed, n):
+    rng = random.Random(seed)
+    cases = []
+    for _ in range(n):
+        letters = rng.sample("ARNDCQEGHILKMFPSTVWY", 8)
+        alpha, filler = "".join(letters[:rng.randint(2, 5)]), "".join(letters[5:])
+        la, lb = rng.randint(1, 36), rng.randint(1, 36)
+        a, b = rand_seq(rng, la, alpha), rand_seq(rng, lb, alpha)
+        if rng.random() < 0.7:
+            # the query ends in a mutated copy of a stretch at the ref's start,
+            # with filler around it: ref_end < query_end while the ref is the
+            # longer sequence, so the reverse pass swaps rows and columns
+            # (kernel.cpp:444-459)
+            core = rand_seq(rng, rng.randint(6, 12), alpha)
+            copy = "".join(rng.choice(alpha) if rng.random() < 0.25 else x for x in core)
+            a = core + rand_seq(rng, rng.randint(10, 30), filler)
+            b = rand_seq(rng, rng.randint(4, 12), filler) + copy
+        cases.append((a, b, _asym_table(rng, filler)))
+    return cases
+
+
+@pytest.mark.parametrize("seed", [41, 42, 43])
+def test_oracle_matches_kernel_model_asymmetric_tables(seed):
+    """Asymmetric tables: the forward pass scores M[enc(row)*24 + enc(col)]
+    with rows the longer sequence (kernel.cpp:785-788), the reverse pass with
+    rows the longer end coordinate (kernel.cpp:1021-1030), so when those
+    orientations differ its maximum is not the forward score; begins come from
+    the reverse DP's own maximum (kernel.cpp:3-45, 90-135), 0 when it is 0."""
+    swapped_other_max = 0
+    for a, b, mat in _asym_cases(seed, 140):
+        fn = lambda x, y, m=mat: m[encode_aa(ord(x)) * 24 + encode_aa(ord(y))]
+        sc = ol.aa_scoring(mat, -4, -1)
+        got = ol.align(a, b, sc)
+        mn = max(1, min(len(a), len(b)))
+        T = random.Random(len(a) * 37 + len(b)).choice([mn, mn + 3, max(mn, 33)])
+        s, ea, eb = kernel_model.forward(a, b, T, fn, -4, -1)
+        if s == 0:
+            assert got == (0, 0, 0, 0, 0)
+            continue
+        ba, bb = kernel_model.reverse(a, b, ea, eb, max(1, min(ea, eb)) + T % 3, fn, -4, -1)
+        assert got == (s, ba, ea, bb, eb), (a, b)
+        fwd_rows_ref = not (len(a) < len(b))
+        rev_rows_ref = not (ea < eb)
+        if fwd_rows_ref != rev_rows_ref and ol.reverse_max(a, b, ea, eb, sc) != s:
+            swapped_other_max += 1
+    # the cases where a rule "begins = cells equal to the forward score" fails
+    assert swapped_other_max > 15, swapped_other_max
