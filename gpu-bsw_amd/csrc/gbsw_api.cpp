@@ -36,6 +36,11 @@ struct gbsw_context {
     int num_cu = 0;
     Exec ex;
     gbsw_batch* oneshot = nullptr; // gbsw_align's batch, reused across calls
+    // gbsw_align_noalloc: its own streams and events, and a batch whose device
+    // buffers are views of the device's static scratch (set up at creation)
+    Exec nx;
+    gbsw_batch* nbatch = nullptr;
+    std::string nx_error;          // why the allocation-free set-up failed ("" if it did not)
 };
 
 struct gbsw_batch {
@@ -115,9 +120,10 @@ int alphabet_from_mask(int32_t alphabet, const uint32_t* mask, uint8_t* lut)
 
 namespace gbsw {
 
-// Upload pairs into batch b (new, or one reused: its device buffers only grow).
-static int batch_fill(gbsw_batch* b, gbsw_context* ctx, int32_t alphabet, int64_t n, const char* ref,
-               const uint64_t* ref_off, const char* query, const uint64_t* query_off)
+// Upload pairs into batch b (new, or one reused: its device buffers only grow),
+// with ex's stream and geometry model.
+static int batch_fill(gbsw_batch* b, gbsw_context* ctx, Exec& ex, int32_t alphabet, int64_t n, const char* ref,
+                      const uint64_t* ref_off, const char* query, const uint64_t* query_off)
 {
     if (!ctx || n < 0 || (n > 0 && (!ref_off || !query_off)))
         return fail(GBSW_E_ARGUMENT, "null argument or negative pair count");
@@ -161,7 +167,7 @@ static int batch_fill(gbsw_batch* b, gbsw_context* ctx, int32_t alphabet, int64_
     const uint8_t* rp = (const uint8_t*)ref + (n ? ref_off[0] : 0);
     const uint8_t* qp = (const uint8_t*)query + (n ? query_off[0] : 0);
     job.lut_host = b->lut;
-    job.kmodel = ctx->ex.kmodel;
+    job.kmodel = ex.kmodel;
     size_t sort_bytes = 0;
     int rc = sort_scratch_bytes(n, &sort_bytes);
     if (rc) return rc;
@@ -187,7 +193,7 @@ static int batch_fill(gbsw_batch* b, gbsw_context* ctx, int32_t alphabet, int64_
     // bytes on the device instead of a pass over them on the host.
     uint32_t seen[8] = {0};
     if (e == hipSuccess) {
-        hipStream_t st = ctx->ex.stream;
+        hipStream_t st = ex.stream;
         uint32_t* dmask = b->keys.as<uint32_t>();
         e = zero_u32(dmask, 8, st);
         if (e == hipSuccess) e = byte_presence(b->ref.as<uint8_t>(), ref_bytes, dmask, st);
@@ -224,6 +230,89 @@ static int batch_fill(gbsw_batch* b, gbsw_context* ctx, int32_t alphabet, int64_
 
 } // namespace gbsw
 
+namespace {
+
+// Both passes of every pair of batch b on ex's streams, blocking.
+int batch_align_on(gbsw_batch* b, Exec& ex, const gbsw_scoring* sc, int32_t mode)
+{
+    if (mode != GBSW_PLAIN && mode != GBSW_BATCHED) return fail(GBSW_E_ARGUMENT, "unknown mode");
+    GBSW_HIP_TRY(hipSetDevice(b->ctx->device));
+    b->stats = gbsw_stats{};
+    b->stats.cells = b->job.cells;
+    b->launches.clear();
+    // a resident batch captures its work into a graph; gbsw_align's one-off
+    // batch does not
+    b->job.graph = b->one_shot ? nullptr : &b->graph;
+    int rc = enqueue_job(ex, b->job, sc, mode);
+    if (rc) {
+        (void)hipStreamSynchronize(ex.stream);
+        return rc;
+    }
+    double ms = 0.0;
+    rc = complete_job(ex, b->job, sc, &ms);
+    b->launches = ex.launches;
+    b->stats.device_ms = ms;
+    b->stats.launches = (int32_t)b->launches.size();
+    b->stats.buckets = mode == GBSW_PLAIN ? 1 : (int32_t)b->job.buckets.size();
+    b->stats.graph = ex.graph_state;
+    for (const LaunchRec& L : b->launches) b->stats.main_kernel_ms = std::max(b->stats.main_kernel_ms, L.ms);
+    b->aligned = rc == GBSW_OK;
+    return rc;
+}
+
+// Carve the device scratch of gbsw_align_noalloc for one call of n pairs:
+// tables, then the batch's arrays, then the redo list; what is left holds the
+// strip-boundary rows (7/8) and the split form's bookkeeping (1/8).  Every
+// buffer becomes a view (never grown, never freed).
+int noalloc_carve(gbsw_context* ctx, int64_t n, uint64_t ref_bytes, uint64_t qry_bytes)
+{
+    uint8_t* base = (uint8_t*)noalloc_scratch(ctx->device);
+    if (!base || !ctx->nx.stream)
+        return fail(GBSW_E_DEVICE, "the allocation-free scratch is not set up on this device: " + ctx->nx_error);
+    size_t sort_bytes = 0;
+    if (int rc = sort_scratch_bytes(n, &sort_bytes)) return rc;
+    const size_t np = (size_t)std::max<int64_t>(n, 1);
+    size_t at = 0;
+    auto take = [&](DeviceBuffer& d, size_t bytes) {
+        const size_t p = at;
+        at = (at + bytes + 255) & ~(size_t)255;
+        if (at <= kNoallocScratchBytes) d.view(base + p, bytes);
+    };
+    Exec& x = ctx->nx;
+    gbsw_batch* b = ctx->nbatch;
+    take(x.status, sizeof(int32_t) * kStatusWords);
+    take(x.queue, sizeof(uint32_t) * kXcds * 64);
+    take(x.sub, sizeof(int16_t) * 1024);
+    take(x.lut, 256);
+    take(x.colmax, sizeof(int16_t) * 32);
+    take(b->bucket_of_len, 65536);
+    take(b->ref, std::max<uint64_t>(ref_bytes, 16));
+    take(b->qry, std::max<uint64_t>(qry_bytes, 16));
+    take(b->ref_off, sizeof(uint64_t) * (np + 1));
+    take(b->qry_off, sizeof(uint64_t) * (np + 1));
+    take(b->out, sizeof(int32_t) * 5 * np);
+    take(b->keys, std::max<size_t>(4 * np, 32));
+    take(b->keys_sorted, 4 * np);
+    take(b->vals, 4 * np);
+    take(b->order, 4 * np);
+    take(b->sort_temp, sort_bytes);
+    take(x.redo, sizeof(int32_t) * np);
+    // at least 1 MiB for boundary rows (a wide pair on one workgroup)
+    constexpr size_t kMinRest = (size_t)1 << 20;
+    if (at + kMinRest > kNoallocScratchBytes)
+        return fail(GBSW_E_RANGE, "gbsw_align_noalloc: " + std::to_string(n) + " pairs of " +
+                                      std::to_string(ref_bytes + qry_bytes) + " residue bytes need " +
+                                      std::to_string(at + kMinRest) + " bytes of scratch; the limit is " +
+                                      std::to_string(kNoallocScratchBytes));
+    const size_t rest = kNoallocScratchBytes - at;
+    const size_t split_bytes = (rest / 8) & ~(size_t)255;
+    take(x.split_buf, split_bytes);
+    take(x.bnd, kNoallocScratchBytes - at);
+    return GBSW_OK;
+}
+
+} // namespace
+
 extern "C" {
 
 const char* gbsw_last_error(void) { return gbsw::last_error_text(); }
@@ -258,6 +347,15 @@ int gbsw_context_create(int32_t device, gbsw_context** out)
         delete ctx;
         return rc;
     }
+    // set-up of the allocation-free entry point: the device address of this
+    // device's scratch, its streams and events; a failure here only disables
+    // gbsw_align_noalloc (which then reports it)
+    int nrc = noalloc_setup(device);
+    if (!nrc) nrc = ctx->nx.init_views(device, ctx->num_cu);
+    if (nrc) ctx->nx_error = gbsw_last_error();
+    ctx->nbatch = new gbsw_batch();
+    ctx->nbatch->one_shot = true;
+    set_error("");
     *out = ctx;
     return GBSW_OK;
 }
@@ -297,6 +395,8 @@ int gbsw_context_destroy(gbsw_context* ctx)
     if (!ctx) return GBSW_OK;
     (void)hipSetDevice(ctx->device);
     delete ctx->oneshot;
+    delete ctx->nbatch;
+    ctx->nx.destroy();
     ctx->ex.destroy();
     delete ctx;
     return GBSW_OK;
@@ -309,7 +409,8 @@ int gbsw_batch_create(gbsw_context* ctx, int32_t alphabet, int64_t n, const char
     if (!out) return fail(GBSW_E_ARGUMENT, "out is null");
     *out = nullptr;
     gbsw_batch* b = new gbsw_batch();
-    const int rc = batch_fill(b, ctx, alphabet, n, ref, ref_off, query, query_off);
+    const int rc = ctx ? batch_fill(b, ctx, ctx->ex, alphabet, n, ref, ref_off, query, query_off)
+                       : fail(GBSW_E_ARGUMENT, "context is null");
     if (rc) {
         const std::string err = gbsw_last_error();
         delete b;
@@ -331,30 +432,7 @@ int gbsw_batch_destroy(gbsw_batch* b)
 int gbsw_batch_align(gbsw_batch* b, const gbsw_scoring* sc, int32_t mode)
 {
     if (!b || !sc) return fail(GBSW_E_ARGUMENT, "null argument");
-    if (mode != GBSW_PLAIN && mode != GBSW_BATCHED) return fail(GBSW_E_ARGUMENT, "unknown mode");
-    gbsw_context* ctx = b->ctx;
-    GBSW_HIP_TRY(hipSetDevice(ctx->device));
-    b->stats = gbsw_stats{};
-    b->stats.cells = b->job.cells;
-    b->launches.clear();
-    // a resident batch captures its work into a graph; gbsw_align's one-off
-    // batch does not
-    b->job.graph = b->one_shot ? nullptr : &b->graph;
-    int rc = enqueue_job(ctx->ex, b->job, sc, mode);
-    if (rc) {
-        (void)hipStreamSynchronize(ctx->ex.stream);
-        return rc;
-    }
-    double ms = 0.0;
-    rc = complete_job(ctx->ex, b->job, sc, &ms);
-    b->launches = ctx->ex.launches;
-    b->stats.device_ms = ms;
-    b->stats.launches = (int32_t)b->launches.size();
-    b->stats.buckets = mode == GBSW_PLAIN ? 1 : (int32_t)b->job.buckets.size();
-    b->stats.graph = ctx->ex.graph_state;
-    for (const LaunchRec& L : b->launches) b->stats.main_kernel_ms = std::max(b->stats.main_kernel_ms, L.ms);
-    b->aligned = rc == GBSW_OK;
-    return rc;
+    return batch_align_on(b, b->ctx->ex, sc, mode);
 }
 
 int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity, uint32_t* cigar,
@@ -630,8 +708,25 @@ int gbsw_align(gbsw_context* ctx, const gbsw_scoring* sc, int32_t mode, int64_t 
         ctx->oneshot->one_shot = true;
     }
     gbsw_batch* b = ctx->oneshot;
-    int rc = batch_fill(b, ctx, sc->alphabet, n, ref, ref_off, query, query_off);
+    int rc = batch_fill(b, ctx, ctx->ex, sc->alphabet, n, ref, ref_off, query, query_off);
     if (!rc) rc = gbsw_batch_align(b, sc, mode);
+    if (!rc) rc = gbsw_batch_results(b, score, ref_begin, ref_end, query_begin, query_end);
+    return rc;
+}
+
+int gbsw_align_noalloc(gbsw_context* ctx, const gbsw_scoring* sc, int32_t mode, int64_t n, const char* ref,
+                       const uint64_t* ref_off, const char* query, const uint64_t* query_off, int32_t* score,
+                       int32_t* ref_begin, int32_t* ref_end, int32_t* query_begin, int32_t* query_end)
+{
+    if (!sc || !ctx) return fail(GBSW_E_ARGUMENT, "null argument");
+    if (n < 0 || (n > 0 && (!ref_off || !query_off))) return fail(GBSW_E_ARGUMENT, "null offsets or negative pair count");
+    GBSW_HIP_TRY(hipSetDevice(ctx->device));
+    const uint64_t rb = n ? ref_off[n] - ref_off[0] : 0;
+    const uint64_t qb = n ? query_off[n] - query_off[0] : 0;
+    int rc = noalloc_carve(ctx, n, rb, qb);
+    gbsw_batch* b = ctx->nbatch;
+    if (!rc) rc = batch_fill(b, ctx, ctx->nx, sc->alphabet, n, ref, ref_off, query, query_off);
+    if (!rc) rc = batch_align_on(b, ctx->nx, sc, mode);
     if (!rc) rc = gbsw_batch_results(b, score, ref_begin, ref_end, query_begin, query_end);
     return rc;
 }

@@ -29,14 +29,16 @@ const char* last_error_text() { return g_err.c_str(); }
 
 void DeviceBuffer::release()
 {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && owned) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+    owned = true;
 }
 
 hipError_t DeviceBuffer::reserve(size_t b)
 {
     if (b <= bytes) return hipSuccess;
+    if (!owned) return hipErrorOutOfMemory; // a view never grows
     release();
     hipError_t e = hipMalloc(&ptr, b);
     if (e == hipSuccess) bytes = b;
@@ -177,6 +179,32 @@ int Exec::init(int dev, int ncu)
     if (e == hipSuccess) e = lut.reserve(256);
     if (e == hipSuccess) e = colmax.reserve(sizeof(int16_t) * 32);
     if (e != hipSuccess) return fail(GBSW_E_DEVICE, std::string("stream/scratch setup: ") + hipGetErrorString(e));
+    return GBSW_OK;
+}
+
+int Exec::init_views(int dev, int ncu)
+{
+    device = dev;
+    num_cu = ncu;
+    streams = default_streams();
+    f16 = default_f16();
+    split = default_split();
+    graphs = false;
+    fixed = true;
+    kmodel = default_kmodel();
+    GBSW_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    GBSW_HIP_TRY(hipEventCreate(&ev_start));
+    GBSW_HIP_TRY(hipEventCreate(&ev_stop));
+    GBSW_HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    // every stream a job may use, and the launch events, now: a job then creates nothing
+    if (int rc = ensure_streams(kAuxStreams + 1)) return rc;
+    while (kev.size() < 2 * (size_t)kNoallocLaunchEvents) {
+        hipEvent_t e;
+        GBSW_HIP_TRY(hipEventCreate(&e));
+        kev.push_back(e);
+    }
+    GBSW_HIP_TRY(status_host.reserve(sizeof(int32_t) * kStatusWords));
+    GBSW_HIP_TRY(tables_host.reserve(4096));
     return GBSW_OK;
 }
 
@@ -497,7 +525,8 @@ int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int
                 bnd_words += (size_t)2 * b.grid * kPairsPerWave * (size_t)bnd_rows;
                 b.split_at = split_words;
                 split_words += split_words_of(bk, b);
-            } else {
+            } else if (b.strips > 1) {
+                // (a bucket whose pairs all fit one strip hands no boundary rows on)
                 bnd_words += (size_t)b.grid * kPairsPerWave * (size_t)bnd_rows;
             }
         }
@@ -515,6 +544,18 @@ int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int
             plan[w].kind = with_profile(batched_kind(job.buckets[w], max_sub, ex.f16 && !p8), p8);
             plan[w].grid = grid_for(ex, plan[w].kind, plan[w].K, plan[w].lds,
                                     (job.buckets[w].count + kPairsPerWave - 1) / kPairsPerWave);
+            sizes();
+        }
+        // then the widest multi-strip grids shrink (a persistent grid of any size
+        // drains its queue) until the boundary rows fit
+        while (bnd_words * sizeof(uint32_t) > ex.bnd.bytes) {
+            size_t w = nb;
+            for (size_t i = 0; i < nb; ++i)
+                if (!plan_is_split(plan[i]) && plan[i].strips > 1 && plan[i].grid > 1 &&
+                    (w == nb || plan[i].grid > plan[w].grid))
+                    w = i;
+            if (w == nb) return fail(GBSW_E_RANGE, "strip boundary rows exceed the preallocated scratch");
+            plan[w].grid = std::max(1, plan[w].grid / 2);
             sizes();
         }
     }
@@ -682,9 +723,17 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
         a.order = nullptr;
         a.order_n = job.n;
         const int kf = with_profile(kPlainForward, p8), kr = with_profile(kPlainReverse, p8);
-        const int gf = grid_for(ex, kf, K, lds, nquads);
-        const int gr = grid_for(ex, kr, K, lds, nquads);
-        GBSW_HIP_TRY(ex.bnd.reserve((size_t)std::max(gf, gr) * kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t)));
+        int gf = grid_for(ex, kf, K, lds, nquads);
+        int gr = grid_for(ex, kr, K, lds, nquads);
+        const size_t per_wg = kPairsPerWave * (size_t)a.bnd_rows * sizeof(uint32_t);
+        if (ex.fixed && (size_t)std::max(gf, gr) * per_wg > ex.bnd.bytes) {
+            // preallocated scratch: fewer workgroups stride over the quads
+            const int cap = (int)std::min<size_t>(ex.bnd.bytes / per_wg, INT_MAX);
+            if (cap < 1) return fail(GBSW_E_RANGE, "strip boundary rows exceed the preallocated scratch");
+            gf = std::min(gf, cap);
+            gr = std::min(gr, cap);
+        }
+        GBSW_HIP_TRY(ex.bnd.reserve((size_t)std::max(gf, gr) * per_wg));
         rc = launch_timed(ex, st0, kf, K, gf, lds, a, ex.bnd.ptr);
         if (rc) return rc;
         ex.launches.back().pairs = job.n;
@@ -779,8 +828,14 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
         a.queue = ex.queue.as<uint32_t>();
         GBSW_HIP_TRY(zero_u32(a.queue, 1, ex.stream));
         const int kw = with_profile(kBatchedWide, ex.p8);
-        const int grid = grid_for(ex, kw, K, lds, (count + kPairsPerWave - 1) / kPairsPerWave);
-        GBSW_HIP_TRY(ex.bnd.reserve((size_t)grid * kPairsPerWave * (size_t)a.bnd_rows * 8));
+        int grid = grid_for(ex, kw, K, lds, (count + kPairsPerWave - 1) / kPairsPerWave);
+        const size_t per_wg = kPairsPerWave * (size_t)a.bnd_rows * 8;
+        if (ex.fixed && (size_t)grid * per_wg > ex.bnd.bytes) {
+            const int cap = (int)std::min<size_t>(ex.bnd.bytes / per_wg, INT_MAX);
+            if (cap < 1) return fail(GBSW_E_RANGE, "int32 boundary rows exceed the preallocated scratch");
+            grid = std::min(grid, cap);
+        }
+        GBSW_HIP_TRY(ex.bnd.reserve((size_t)grid * per_wg));
         int rc = launch_timed(ex, ex.stream, kw, K, grid, lds, a, ex.bnd.ptr);
         if (rc) return rc;
         ex.launches.back().pairs = count;

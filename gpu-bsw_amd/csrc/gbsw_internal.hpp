@@ -22,15 +22,25 @@ int fail(int code, const std::string& msg);
             return ::gbsw::fail(GBSW_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device memory owned by the buffer (grown by reserve), or a view of memory
+// owned elsewhere (view(): reserve never allocates and fails beyond its size).
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
     ~DeviceBuffer() { release(); }
     void release();
     hipError_t reserve(size_t b);
+    void view(void* p, size_t b)
+    {
+        release();
+        owned = false;
+        ptr = p;
+        bytes = b;
+    }
     template <class T> T* as() const { return (T*)ptr; }
 };
 
@@ -88,11 +98,16 @@ struct Exec {
                                    // XCCs: the split form stays off for this Exec
     bool late_results = false;     // last complete_job wrote results after the job's own copies
                                    // (int32 recompute or unsplit rerun)
-    bool fixed = false;            // buffers preallocated (serving slot): never grown by a job
+    bool fixed = false;            // buffers preallocated (serving slot, or views of the
+                                   // allocation-free scratch): never grown by a job; grids
+                                   // shrink to fit its boundary rows
     bool graphs = true;            // batched form of a resident batch replays a captured hipGraph
     int graph_state = 0;           // last job: 0 eager, 1 captured, 2 replayed
     int kmodel = 1;                // length -> geometry model of new batches (kModel*)
     int init(int device, int num_cu);
+    // streams, events and pinned staging only; every device buffer is a view set
+    // by the caller (the allocation-free entry point), launch events pre-created
+    int init_views(int device, int num_cu);
     void destroy();
 };
 
@@ -181,6 +196,16 @@ int sort_scratch_bytes(int64_t n, size_t* bytes);
 // Validate a scoring for a job and build its substitution table; *p8: the job
 // can use the int8 profile.
 int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8);
+
+// Allocation-free entry point (gbsw_align_noalloc): the device scratch is one
+// static array per device (gbsw_noalloc.hip) of kNoallocScratchBytes.
+constexpr size_t kNoallocScratchBytes = (size_t)256 << 20;
+constexpr int kNoallocLaunchEvents = 128; // event pairs pre-created for its launches
+// Set-up, once per device with that device current: the device address of the
+// device's scratch array (hipGetSymbolAddress).  Thread-safe.
+int noalloc_setup(int device);
+// That address (nullptr before noalloc_setup succeeded on the device).
+void* noalloc_scratch(int device);
 
 // Enqueue a whole alignment job (binning if batched, kernels) on ex.stream.
 // Asynchronous: ex.ev_stop is recorded after the last kernel and the status words

@@ -113,7 +113,7 @@ EXPORTED = (
     "gbsw_version",
     "gbsw_server_config_default", "gbsw_server_create", "gbsw_server_destroy", "gbsw_server_submit",
     "gbsw_server_wait", "gbsw_server_poll", "gbsw_server_stats", "gbsw_host_alloc", "gbsw_host_free",
-    "gbsw_context_set_option", "gbsw_batch_traceback",
+    "gbsw_context_set_option", "gbsw_batch_traceback", "gbsw_align_noalloc",
 )
 OPT_STREAMS, OPT_F16, OPT_GEOMETRY, OPT_SPLIT, OPT_GRAPHS = 1, 2, 3, 4, 5
 # launch kinds (gbsw_launch_stat.kind): plain forward, plain reverse, batched
@@ -153,6 +153,7 @@ def load_library():
     lib.gbsw_batch_launch_stats.argtypes = [vp, i32, ctypes.POINTER(LaunchStat)]
     lib.gbsw_batch_traceback.argtypes = [vp, ctypes.POINTER(Scoring), i64, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
     lib.gbsw_align.argtypes = [vp, ctypes.POINTER(Scoring), i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.gbsw_align_noalloc.argtypes = [vp, ctypes.POINTER(Scoring), i32, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.gbsw_device_memory.argtypes = [i32, ctypes.POINTER(ctypes.c_uint64)]
     u64 = ctypes.c_uint64
     lib.gbsw_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
@@ -237,7 +238,9 @@ class Context:
         qb, qo = pack(queries)
         return self.align_packed(rb, ro, qb, qo, scoring, mode)
 
-    def align_packed(self, rb, ro, qb, qo, scoring, mode="batched"):
+    def align_packed(self, rb, ro, qb, qo, scoring, mode="batched", noalloc=False):
+        """Packed buffers and n+1 offsets in, dict of five int32 arrays out
+        (gbsw_align, or gbsw_align_noalloc when noalloc is set)."""
         n = len(ro) - 1
         if len(qo) - 1 != n:
             raise ValueError("ref and query counts differ")
@@ -247,8 +250,9 @@ class Context:
         qb = np.ascontiguousarray(qb, dtype=np.uint8)
         ro = np.ascontiguousarray(ro, dtype=np.uint64)
         qo = np.ascontiguousarray(qo, dtype=np.uint64)
-        _check(self._lib.gbsw_align(self._h, ctypes.byref(scoring), MODES.get(mode, mode), n, rb.ctypes.data,
-                                    ro.ctypes.data, qb.ctypes.data, qo.ctypes.data, *ptrs))
+        fn = self._lib.gbsw_align_noalloc if noalloc else self._lib.gbsw_align
+        _check(fn(self._h, ctypes.byref(scoring), MODES.get(mode, mode), n, rb.ctypes.data, ro.ctypes.data,
+                  qb.ctypes.data, qo.ctypes.data, *ptrs))
         return {f: out[k][:n].copy() for k, f in enumerate(FIELDS)}
 
 

@@ -192,6 +192,25 @@ int gbsw_align(gbsw_context* ctx, const gbsw_scoring* scoring, int32_t mode, int
                const uint64_t* query_off, int32_t* score, int32_t* ref_begin, int32_t* ref_end,
                int32_t* query_begin, int32_t* query_end);
 
+/* gbsw_align without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device or pinned memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * gbsw_align, but every device buffer of the call -- sequences, offsets,
+ * results, binning and sort arrays, strip-boundary rows, counters -- is carved
+ * from one static device array per device of 256 MiB (set up by
+ * gbsw_context_create; each device the library runs on holds it until the
+ * process ends), and the call makes no hipMalloc, hipFree or pinned
+ * allocation.  A call whose sequences and per-pair arrays (offsets, results,
+ * binning and sort arrays: about 60 bytes per pair) leave less than 1 MiB of
+ * the array free is
+ * rejected with GBSW_E_RANGE before any device work; boundary rows that do not
+ * fit the rest run on fewer workgroups.  All contexts of one device share its
+ * array: calls that overlap on one device race (serialise them). */
+int gbsw_align_noalloc(gbsw_context* ctx, const gbsw_scoring* scoring, int32_t mode, int64_t n,
+                       const char* ref, const uint64_t* ref_off, const char* query,
+                       const uint64_t* query_off, int32_t* score, int32_t* ref_begin, int32_t* ref_end,
+                       int32_t* query_begin, int32_t* query_end);
+
 /* Total device memory in bytes (driver.cpp:10-14, get_tot_gpu_mem). */
 int gbsw_device_memory(int32_t device, uint64_t* bytes);
 
