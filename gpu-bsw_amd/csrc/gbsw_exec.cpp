@@ -151,6 +151,7 @@ KernelArgs base_args(const Exec& ex, const Job& job, const gbsw_scoring* sc)
     a.gap_open = sc->gap_open;
     a.gap_extend = sc->gap_extend;
     a.sat_limit = 32767 - ex.max_sub;
+    a.asym = table_is_asymmetric(sc) ? 1 : 0;
     a.bnd_rows = std::max(1, job.max_rows);
     return a;
 }
@@ -410,6 +411,15 @@ int sort_scratch_bytes(int64_t n, size_t* bytes)
     return GBSW_OK;
 }
 
+bool table_is_asymmetric(const gbsw_scoring* sc)
+{
+    if (sc->alphabet != GBSW_PROTEIN) return false;
+    for (int r = 0; r < 24; ++r)
+        for (int c = 0; c < r; ++c)
+            if (sc->matrix[r * 24 + c] != sc->matrix[c * 24 + r]) return true;
+    return false;
+}
+
 int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8)
 {
     if (sc->alphabet != job.alphabet) return fail(GBSW_E_ARGUMENT, "scoring alphabet differs from the batch alphabet");
@@ -426,12 +436,6 @@ int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>
             sub[(size_t)k] = sc->matrix[k];
             mx = std::max<int>(mx, sc->matrix[k]);
         }
-        // the reverse pass may take the other sequence as rows (kernel.cpp:444-459):
-        // with an asymmetric table its maximum need not be the forward score
-        for (int r = 0; r < 24; ++r)
-            for (int c = 0; c < r; ++c)
-                if (sc->matrix[r * 24 + c] != sc->matrix[c * 24 + r])
-                    return fail(GBSW_E_SCORING, "the substitution matrix must be symmetric");
     }
     for (int16_t v : sub)
         if (v < -16000 || v > 16000) return fail(GBSW_E_SCORING, "substitution scores must lie in [-16000, 16000]");
@@ -479,7 +483,7 @@ size_t split_words_of(const Bucket& bk, const BucketPlan& b)
 }
 
 // Kernels, grids and scratch of every bucket (host only; reserves the buffers).
-int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int64_t bnd_rows,
+int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, bool asym, int kclamp, int64_t bnd_rows,
                  std::vector<BucketPlan>& plan, std::vector<size_t>& order)
 {
     const size_t nb = job.buckets.size();
@@ -500,7 +504,9 @@ int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, int kclamp, int
         // workgroup busy for two rounds (on the 200k-pair DNA mix: 1765 vs 827
         // GCUPS at 1.4 quads per workgroup; 988 vs 1275 on protein at 2.7).
         const int64_t full = (int64_t)std::max(1, kernel_occupancy(b.kind, b.K, b.lds)) * ex.num_cu;
-        if (ex.split && !ex.split_misplaced && b.strips > 1 && nquads < 2 * full && kernel_has(skind, b.K)) {
+        // (an asymmetric table may need a third pass per pair, which the split
+        // form's strip items do not schedule)
+        if (ex.split && !ex.split_misplaced && !asym && b.strips > 1 && nquads < 2 * full && kernel_has(skind, b.K)) {
             // every XCD's counter needs workgroups: a multiple of kXcds, no more
             // than the items a pipelined quad can keep busy, and two boundary
             // slots per workgroup
@@ -748,7 +754,7 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     }
     std::vector<BucketPlan> plan;
     std::vector<size_t> order;
-    rc = plan_batched(ex, job, max_sub, p8, kclamp, a.bnd_rows, plan, order);
+    rc = plan_batched(ex, job, max_sub, p8, a.asym != 0, kclamp, a.bnd_rows, plan, order);
     if (rc) return rc;
     const size_t sub_bytes = sizeof(int16_t) * sub.size();
     if (!ex.graphs || ex.fixed || !job.graph) {

@@ -193,6 +193,9 @@ int batched_kind(const Bucket& bk, int32_t max_sub, bool allow_f16);
 // Scratch bytes of the radix sort for n pairs.
 int sort_scratch_bytes(int64_t n, size_t* bytes);
 
+// A protein table with M[r][c] != M[c][r] for some pair (the reverse pass may
+// then need its own maximum, kernel.cpp:444-459, 1021-1030).
+bool table_is_asymmetric(const gbsw_scoring* sc);
 // Validate a scoring for a job and build its substitution table; *p8: the job
 // can use the int8 profile.
 int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8);

@@ -101,6 +101,8 @@ struct KernelArgs {
     int32_t gap_open;          // <= 0
     int32_t gap_extend;        // <= 0
     int32_t sat_limit;         // scores >= this may have saturated int16
+    int32_t asym;              // 1: sub[] is not symmetric (the reverse pass then finds its own maximum
+                               //    when its orientation differs from the forward pass's)
     SplitArgs split;           // k_split only
 };
 

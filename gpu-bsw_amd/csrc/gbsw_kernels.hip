@@ -931,12 +931,78 @@ __device__ __forceinline__ Task reverse_task(const KernelArgs& a, int64_t p, con
     return t;
 }
 
+// The reverse pass takes the other sequence as rows than the forward pass
+// (kernel.cpp:248-259 decide by the full lengths, 444-459 by the end coordinates).
+__device__ __forceinline__ bool reverse_swaps(const KernelArgs& a, int64_t p, const FwdOut& f)
+{
+    const int32_t la = (int32_t)(a.ref_off[p + 1] - a.ref_off[p]);
+    const int32_t lb = (int32_t)(a.qry_off[p + 1] - a.qry_off[p]);
+    return !(la < lb) != !(f.ref_end < f.query_end);
+}
+
+// The whole reversed prefix box A[0:ref_end) x B[0:query_end), no extent bound.
+__device__ __forceinline__ Task reverse_task_full(const KernelArgs& a, int64_t p, const FwdOut& f)
+{
+    Task t;
+    t.rev = 1;
+    const uint8_t* A = a.ref + a.ref_off[p];
+    const uint8_t* B = a.qry + a.qry_off[p];
+    if (f.ref_end < f.query_end) { t.rows = B; t.nr = f.query_end; t.cols = A; t.nc = f.ref_end; } // 444-451
+    else { t.rows = A; t.nr = f.ref_end; t.cols = B; t.nc = f.query_end; }                        // 452-459
+    t.nr_full = t.nr;
+    t.nc_full = t.nc;
+    return t;
+}
+
+// Reverse target of a pair.  With a symmetric table, or an asymmetric one whose
+// reverse pass keeps the forward orientation, the reversed prefix box holds the
+// same alignments with the same scores, so its maximum is the forward score S
+// and the extent bound applies.  With an asymmetric table and the orientation
+// swapped, the reverse DP scores the transposed table and its maximum (which
+// decides the begins, kernel.cpp:1021-1030, 3-45) is found first, by a
+// forward-style pass over the whole reversed box.  A target of 0 gives begins
+// (0, 0), the reference's initial per-thread coordinates.
+struct RevPlan {
+    Task t;
+    int32_t target;
+    bool saturated; // that maximum may have saturated int16: recomputed with int32
+};
+template <class A, int K, bool P8>
+__device__ __forceinline__ RevPlan reverse_plan(const KernelArgs& a, const Smem& sm, int64_t p, const FwdOut& f,
+                                                typename A::B* bnd_g)
+{
+    RevPlan r;
+    r.t = reverse_task(a, p, f, column_bound(a, sm, p, f));
+    r.target = f.score;
+    r.saturated = false;
+    if (a.asym) {
+        const bool gen = p >= 0 && f.score > 0 && reverse_swaps(a, p, f);
+        if (__ballot(gen) != 0) { // wave-uniform: every group runs the pass (empty tasks idle)
+            const Task tm = gen ? reverse_task_full(a, p, f) : empty_task(a, 1);
+            const PassResult rm = run_pass<A, K, false, P8>(a, sm, tm, 0, bnd_g);
+            if (gen) {
+                r.target = (int32_t)(rm.key >> 40);
+                r.t = r.target > 0 ? tm : empty_task(a, 1);
+                if (r.target >= a.sat_limit) {
+                    r.saturated = true;
+                    r.t = empty_task(a, 1);
+                    if ((threadIdx.x & 15) == 0) {
+                        const int32_t slot = atomicAdd(a.status + kStatusSaturated, 1);
+                        a.redo[slot] = (int32_t)p;
+                    }
+                }
+            }
+        }
+    }
+    return r;
+}
+
 __device__ __forceinline__ void reverse_result(const KernelArgs& a, int64_t p, const FwdOut& f, const Task& t,
-                                               uint64_t key, int32_t* ref_begin, int32_t* query_begin)
+                                               int32_t target, uint64_t key, int32_t* ref_begin, int32_t* query_begin)
 {
     *ref_begin = 0;
     *query_begin = 0;
-    if (p < 0 || f.score <= 0) return;
+    if (p < 0 || f.score <= 0 || target <= 0) return;
     if (key == 0) { atomicOr(a.status + kStatusRevMiss, 1); return; }
     const int32_t col = (int32_t)(0xFFFFFu - (uint32_t)((key >> 20) & 0xFFFFFu));
     const int32_t row = (int32_t)(0xFFFFFu - (uint32_t)(key & 0xFFFFFu));
@@ -1024,11 +1090,11 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
             f.ref_end = a.out[kOutRefEnd * a.n + p];
             f.query_end = a.out[kOutQueryEnd * a.n + p];
         }
-        const Task t = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult r = run_pass<Packed16, K, true, P8>(a, sm, t, f.score, bnd_g);
+        const RevPlan rp = reverse_plan<Packed16, K, P8>(a, sm, p, f, bnd_g);
+        const PassResult r = run_pass<Packed16, K, true, P8>(a, sm, rp.t, rp.target, bnd_g);
         int32_t rb, qb;
-        reverse_result(a, p, f, t, r.key, &rb, &qb);
-        if (p >= 0 && l == 0 && f.score >= 0) {
+        reverse_result(a, p, f, rp.t, rp.target, r.key, &rb, &qb);
+        if (p >= 0 && l == 0 && f.score >= 0 && !rp.saturated) {
             a.out[kOutRefBegin * a.n + p] = rb;
             a.out[kOutQueryBegin * a.n + p] = qb;
         }
@@ -1057,11 +1123,11 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
         const Task tf = forward_task(a, p);
         const PassResult rf = run_pass<A, K, false, P8>(a, sm, tf, 0, bnd_g);
         const FwdOut f = forward_result(a, p, rf.key);
-        const Task tr = reverse_task(a, p, f, column_bound(a, sm, p, f));
-        const PassResult rr = run_pass<A, K, true, P8>(a, sm, tr, f.score, bnd_g);
+        const RevPlan rp = reverse_plan<A, K, P8>(a, sm, p, f, bnd_g);
+        const PassResult rr = run_pass<A, K, true, P8>(a, sm, rp.t, rp.target, bnd_g);
         int32_t rb, qb;
-        reverse_result(a, p, f, tr, rr.key, &rb, &qb);
-        if (p >= 0 && l == 0 && f.score >= 0) {
+        reverse_result(a, p, f, rp.t, rp.target, rr.key, &rb, &qb);
+        if (p >= 0 && l == 0 && f.score >= 0 && !rp.saturated) {
             a.out[kOutScore * a.n + p] = f.score;
             a.out[kOutRefEnd * a.n + p] = f.ref_end;
             a.out[kOutQueryEnd * a.n + p] = f.query_end;
@@ -1189,7 +1255,7 @@ __global__ __launch_bounds__(64) void k_split(KernelArgs a)
         if (rev && before == (uint32_t)sp.strips - 1) {
             const uint64_t rk = p >= 0 ? __hip_atomic_load(rkey + ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             int32_t rb, qb;
-            reverse_result(a, p, f, t, rk, &rb, &qb);
+            reverse_result(a, p, f, t, f.score, rk, &rb, &qb); // (the host never splits asymmetric tables)
             if (p >= 0 && l == 0 && f.score >= 0) {
                 a.out[kOutScore * a.n + p] = f.score;
                 a.out[kOutRefEnd * a.n + p] = f.ref_end;

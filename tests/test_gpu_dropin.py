@@ -151,3 +151,21 @@ def test_results_beyond_short_are_range_errors(caller, tmp_path):
     want[:, 4] = np.clip(exp["query_end"], -32768, 32767) - 1
     assert exp["score"].max() > 32767
     assert_rows(rows, want, "clamped")
+
+
+def test_kernel_driver_aa_asymmetric_table(caller, tmp_path):
+    """kernel_driver_aa with an asymmetric 576-entry table (the reference takes
+    any table, driver.cpp:304-308): the five columns equal the oracle's."""
+    from test_gpu_asymmetric import skewed_blosum
+    w = synth.plain_protein(3000, seed=0x5EED0503)
+    table = skewed_blosum(11)
+    fr, fq, fm = tmp_path / "refs.txt", tmp_path / "queries.txt", tmp_path / "m.txt"
+    write_lines(fr, w.ref, w.ref_off)
+    write_lines(fq, w.query, w.query_off)
+    fm.write_text(" ".join(str(v) for v in table) + "\n")
+    out = run_caller([caller, "fileaa", str(fr), str(fq), str(fm), "-6", "-1"])
+    assert out.returncode == 0, out.stderr[-2000:]
+    rows, status = columns(out.stdout)
+    assert status == 0
+    exp = ol.align_batch(w.ref, w.ref_off, w.query, w.query_off, ol.aa_scoring(table, -6, -1))
+    assert_rows(rows, expected_columns(exp), "kernel_driver_aa asymmetric table")

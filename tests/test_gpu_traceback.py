@@ -150,21 +150,6 @@ def test_protein_paths_with_another_symmetric_table(ctx):
                 "protein symmetric table")
 
 
-def test_asymmetric_tables_are_rejected(ctx):
-    # the reverse pass may swap rows and columns (kernel.cpp:444-459); an
-    # asymmetric table would make its maximum differ from the forward score
-    mat = list(BLOSUM62)
-    mat[0 * 24 + 1] += 1
-    rb, ro, qb, qo = pack([("ARND", "ARNE")])
-    b = ctx.batch(gbsw.PROTEIN, rb, ro, qb, qo)
-    try:
-        with pytest.raises(gbsw.GbswError) as e:
-            b.align(gbsw.protein_scoring(mat, -6, -1), "batched")
-        assert e.value.code == -3
-    finally:
-        b.close()
-
-
 def test_plain_form_results_give_the_same_paths(ctx):
     w = synth.plain_dna(800, seed=24)
     check_paths(ctx, gbsw.DNA, w.ref, w.ref_off, w.query, w.query_off, gbsw.dna_scoring(*w.scoring),
