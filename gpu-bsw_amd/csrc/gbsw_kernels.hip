@@ -960,41 +960,41 @@ __device__ __forceinline__ Task reverse_task_full(const KernelArgs& a, int64_t p
 // and the extent bound applies.  With an asymmetric table and the orientation
 // swapped, the reverse DP scores the transposed table and its maximum (which
 // decides the begins, kernel.cpp:1021-1030, 3-45) is found first, by a
-// forward-style pass over the whole reversed box.  A target of 0 gives begins
-// (0, 0), the reference's initial per-thread coordinates.
+// forward-style pass over the whole reversed box (run by the kernels through
+// their one forward-pass call site, so that no second copy of the pass is
+// inlined).  A target of 0 gives begins (0, 0), the reference's initial
+// per-thread coordinates.
 struct RevPlan {
     Task t;
     int32_t target;
+    bool gen;       // this group's pair needs the maximum of its reversed box first
     bool saturated; // that maximum may have saturated int16: recomputed with int32
 };
-template <class A, int K, bool P8>
-__device__ __forceinline__ RevPlan reverse_plan(const KernelArgs& a, const Smem& sm, int64_t p, const FwdOut& f,
-                                                typename A::B* bnd_g)
+__device__ __forceinline__ RevPlan reverse_plan(const KernelArgs& a, const Smem& sm, int64_t p, const FwdOut& f)
 {
     RevPlan r;
     r.t = reverse_task(a, p, f, column_bound(a, sm, p, f));
     r.target = f.score;
+    r.gen = a.asym && p >= 0 && f.score > 0 && reverse_swaps(a, p, f);
     r.saturated = false;
-    if (a.asym) {
-        const bool gen = p >= 0 && f.score > 0 && reverse_swaps(a, p, f);
-        if (__ballot(gen) != 0) { // wave-uniform: every group runs the pass (empty tasks idle)
-            const Task tm = gen ? reverse_task_full(a, p, f) : empty_task(a, 1);
-            const PassResult rm = run_pass<A, K, false, P8>(a, sm, tm, 0, bnd_g);
-            if (gen) {
-                r.target = (int32_t)(rm.key >> 40);
-                r.t = r.target > 0 ? tm : empty_task(a, 1);
-                if (r.target >= a.sat_limit) {
-                    r.saturated = true;
-                    r.t = empty_task(a, 1);
-                    if ((threadIdx.x & 15) == 0) {
-                        const int32_t slot = atomicAdd(a.status + kStatusSaturated, 1);
-                        a.redo[slot] = (int32_t)p;
-                    }
-                }
-            }
+    if (r.gen) r.t = reverse_task_full(a, p, f);
+    return r;
+}
+// The forward-style pass over the reversed box (r.t, or an empty task for
+// groups that do not need it) returned `key`: its score is the target.
+__device__ __forceinline__ void reverse_plan_finish(const KernelArgs& a, int64_t p, uint64_t key, RevPlan& r)
+{
+    if (!r.gen) return;
+    r.target = (int32_t)(key >> 40);
+    if (r.target <= 0) r.t = empty_task(a, 1);
+    if (r.target >= a.sat_limit) {
+        r.saturated = true;
+        r.t = empty_task(a, 1);
+        if ((threadIdx.x & 15) == 0) {
+            const int32_t slot = atomicAdd(a.status + kStatusSaturated, 1);
+            a.redo[slot] = (int32_t)p;
         }
     }
-    return r;
 }
 
 __device__ __forceinline__ void reverse_result(const KernelArgs& a, int64_t p, const FwdOut& f, const Task& t,
@@ -1090,7 +1090,11 @@ __global__ __launch_bounds__(64) void k_plain_reverse(KernelArgs a)
             f.ref_end = a.out[kOutRefEnd * a.n + p];
             f.query_end = a.out[kOutQueryEnd * a.n + p];
         }
-        const RevPlan rp = reverse_plan<Packed16, K, P8>(a, sm, p, f, bnd_g);
+        RevPlan rp = reverse_plan(a, sm, p, f);
+        if (__ballot(rp.gen) != 0) { // wave-uniform; groups without it idle on an empty task
+            const PassResult rm = run_pass<Packed16, K, false, P8>(a, sm, rp.gen ? rp.t : empty_task(a, 1), 0, bnd_g);
+            reverse_plan_finish(a, p, rm.key, rp);
+        }
         const PassResult r = run_pass<Packed16, K, true, P8>(a, sm, rp.t, rp.target, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, rp.t, rp.target, r.key, &rb, &qb);
@@ -1120,10 +1124,25 @@ __global__ __launch_bounds__(64) void k_batched(KernelArgs a)
         q = (uint32_t)__builtin_amdgcn_readfirstlane((int)q);
         if ((int64_t)q >= nquads) break;
         const int64_t p = slot_pair(a, (int64_t)q * kPairsPerWave + g);
-        const Task tf = forward_task(a, p);
-        const PassResult rf = run_pass<A, K, false, P8>(a, sm, tf, 0, bnd_g);
-        const FwdOut f = forward_result(a, p, rf.key);
-        const RevPlan rp = reverse_plan<A, K, P8>(a, sm, p, f, bnd_g);
+        // the forward pass, then (asymmetric tables, wave-uniform) the
+        // forward-style pass over the reversed boxes that need their own maximum:
+        // one call site, so the pass is inlined once
+        Task tf = forward_task(a, p);
+        FwdOut f{0, 0, 0};
+        RevPlan rp;
+#pragma nounroll
+        for (int fp = 0;; ++fp) {
+            const PassResult rf = run_pass<A, K, false, P8>(a, sm, tf, 0, bnd_g);
+            if (fp == 0) {
+                f = forward_result(a, p, rf.key);
+                rp = reverse_plan(a, sm, p, f);
+                if (__ballot(rp.gen) == 0) break;
+                tf = rp.gen ? rp.t : empty_task(a, 1);
+            } else {
+                reverse_plan_finish(a, p, rf.key, rp);
+                break;
+            }
+        }
         const PassResult rr = run_pass<A, K, true, P8>(a, sm, rp.t, rp.target, bnd_g);
         int32_t rb, qb;
         reverse_result(a, p, f, rp.t, rp.target, rr.key, &rb, &qb);
