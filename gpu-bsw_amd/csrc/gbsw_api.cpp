@@ -50,6 +50,7 @@ struct gbsw_batch {
     DeviceBuffer bucket_of_len, keys, keys_sorted, vals, sort_temp;
     std::vector<int32_t> len_ref, len_qry;
     uint8_t lut[256];
+    uint8_t code_of[24]; // protein: reference code of each residue code present
     gbsw_stats stats{};
     std::vector<LaunchRec> launches; // of the last gbsw_batch_align
     JobGraph graph;
@@ -95,8 +96,11 @@ int build_alphabet(int32_t alphabet, const uint8_t* const* bytes, const uint64_t
 }
 
 // build_alphabet from the set of byte values present (bit b of mask[b / 32]),
-// as scanned on the device after the upload.
-int alphabet_from_mask(int32_t alphabet, const uint32_t* mask, uint8_t* lut)
+// as scanned on the device after the upload.  Protein: only the reference codes
+// that occur get a residue code (in code order), so the kernels' profiles hold
+// no rows for absent residues; code_of[k] is the reference code (row of the
+// caller's 24x24 matrix) of residue code k.
+int alphabet_from_mask(int32_t alphabet, const uint32_t* mask, uint8_t* lut, uint8_t* code_of)
 {
     auto present = [&](int c) { return (mask[c >> 5] >> (c & 31)) & 1u; };
     std::memset(lut, 0xFF, 256);
@@ -104,10 +108,21 @@ int alphabet_from_mask(int32_t alphabet, const uint32_t* mask, uint8_t* lut)
         for (int c = 91; c < 256; ++c)
             if (present(c))
                 return fail(GBSW_E_RESIDUE, "protein residue byte >= 91 (the reference encoding table has 91 entries)");
-        uint8_t none[1] = {0};
-        const uint8_t* bufs[1] = {none};
-        const uint64_t lens[1] = {0};
-        return build_alphabet(alphabet, bufs, lens, 0, lut);
+        uint8_t enc[256];
+        build_alphabet(alphabet, nullptr, nullptr, 0, enc);
+        bool used[24] = {false};
+        for (int c = 0; c < 91; ++c)
+            if (present(c)) used[enc[c]] = true;
+        uint8_t compact[24];
+        int A = 0;
+        for (int k = 0; k < 24; ++k)
+            if (used[k]) {
+                compact[k] = (uint8_t)A;
+                code_of[A++] = (uint8_t)k;
+            }
+        if (A == 0) code_of[A++] = 0; // an empty batch: one code
+        for (int c = 0; c < 91; ++c) lut[c] = used[enc[c]] ? compact[enc[c]] : 0xFF;
+        return A;
     }
     int A = 0;
     for (int c = 0; c < 256; ++c)
@@ -205,9 +220,10 @@ static int batch_fill(gbsw_batch* b, gbsw_context* ctx, Exec& ex, int32_t alphab
         return fail(e == hipErrorOutOfMemory ? GBSW_E_MEMORY : GBSW_E_DEVICE,
                     std::string("batch upload: ") + hipGetErrorString(e));
     {
-        const int A = alphabet_from_mask(alphabet, seen, b->lut);
+        const int A = alphabet_from_mask(alphabet, seen, b->lut, b->code_of);
         if (A < 0) return A;
         job.alpha = A;
+        job.code_of = alphabet == GBSW_PROTEIN ? b->code_of : nullptr;
     }
     count_buckets(job, b->len_ref.data(), b->len_qry.data());
     e = hipMemcpy(b->bucket_of_len.ptr, bucket_table(job.alpha, job.kmodel).data(), 65536, hipMemcpyHostToDevice);

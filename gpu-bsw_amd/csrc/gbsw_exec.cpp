@@ -151,7 +151,7 @@ KernelArgs base_args(const Exec& ex, const Job& job, const gbsw_scoring* sc)
     a.gap_open = sc->gap_open;
     a.gap_extend = sc->gap_extend;
     a.sat_limit = 32767 - ex.max_sub;
-    a.asym = table_is_asymmetric(sc) ? 1 : 0;
+    a.asym = table_is_asymmetric(sc, job) ? 1 : 0;
     a.bnd_rows = std::max(1, job.max_rows);
     return a;
 }
@@ -411,12 +411,14 @@ int sort_scratch_bytes(int64_t n, size_t* bytes)
     return GBSW_OK;
 }
 
-bool table_is_asymmetric(const gbsw_scoring* sc)
+bool table_is_asymmetric(const gbsw_scoring* sc, const Job& job)
 {
     if (sc->alphabet != GBSW_PROTEIN) return false;
-    for (int r = 0; r < 24; ++r)
-        for (int c = 0; c < r; ++c)
-            if (sc->matrix[r * 24 + c] != sc->matrix[c * 24 + r]) return true;
+    for (int r = 0; r < job.alpha; ++r)
+        for (int c = 0; c < r; ++c) {
+            const int cr = job.code_of ? job.code_of[r] : r, cc = job.code_of ? job.code_of[c] : c;
+            if (sc->matrix[cr * 24 + cc] != sc->matrix[cc * 24 + cr]) return true;
+        }
     return false;
 }
 
@@ -432,10 +434,15 @@ int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>
             for (int c = 0; c < A; ++c) sub[(size_t)(r * A + c)] = r == c ? sc->match : sc->mismatch;
         mx = std::max<int>(sc->match, sc->mismatch);
     } else {
-        for (int k = 0; k < 576; ++k) {
-            sub[(size_t)k] = sc->matrix[k];
-            mx = std::max<int>(mx, sc->matrix[k]);
-        }
+        // the caller's table indexed enc(row)*24 + enc(col) (kernel.cpp:785-788),
+        // restricted to the residue codes of the batch
+        for (int r = 0; r < A; ++r)
+            for (int c = 0; c < A; ++c) {
+                const int cr = job.code_of ? job.code_of[r] : r, cc = job.code_of ? job.code_of[c] : c;
+                const int16_t v = sc->matrix[cr * 24 + cc];
+                sub[(size_t)(r * A + c)] = v;
+                mx = std::max<int>(mx, v);
+            }
     }
     for (int16_t v : sub)
         if (v < -16000 || v > 16000) return fail(GBSW_E_SCORING, "substitution scores must lie in [-16000, 16000]");

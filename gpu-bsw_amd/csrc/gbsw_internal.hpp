@@ -140,6 +140,8 @@ struct Job {
     const uint64_t* qry_off = nullptr;
     int32_t* out = nullptr;          // 5*n int32
     int32_t alpha = 0;               // residue codes
+    const uint8_t* code_of = nullptr; // protein: reference code (0..23) of each residue code;
+                                      // nullptr: residue code k is reference code k (alpha 24)
     int kmodel = 1;                  // length -> geometry model (kModel*), fixed at creation
     const uint8_t* lut_host = nullptr; // 256 entries
     int32_t max_rows = 0, max_cols = 0;
@@ -195,7 +197,7 @@ int sort_scratch_bytes(int64_t n, size_t* bytes);
 
 // A protein table with M[r][c] != M[c][r] for some pair (the reverse pass may
 // then need its own maximum, kernel.cpp:444-459, 1021-1030).
-bool table_is_asymmetric(const gbsw_scoring* sc);
+bool table_is_asymmetric(const gbsw_scoring* sc, const Job& job);
 // Validate a scoring for a job and build its substitution table; *p8: the job
 // can use the int8 profile.
 int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8);
