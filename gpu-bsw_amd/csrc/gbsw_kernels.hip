@@ -364,20 +364,21 @@ __device__ __forceinline__ void prof_slot(int j, int& v, int& k)
     }
 }
 
-// Max over K registers as a balanced tree (depth log2 K instead of a chain).
+// Max over K registers with ceil((K - 1) / 2) three-input maxima: a queue, each
+// maximum takes the three oldest entries and appends its result (depth about
+// log3 K).  Op j reads entries 3j .. 3j + 2 of the queue, which then holds
+// K - 2j entries; only a last op with two entries left is a two-input maximum.
 template <class A, int K>
 __device__ __forceinline__ typename A::V tree_max(const typename A::V (&h)[K])
 {
-    // ternary tree: ceil((K - 1) / 2) three-input maxima
-    typename A::V t[K];
+    constexpr int NOPS = (K - 1 + 1) / 2;
+    typename A::V q[K + NOPS];
 #pragma unroll
-    for (int k = 0; k < K; ++k) t[k] = h[k];
+    for (int k = 0; k < K; ++k) q[k] = h[k];
 #pragma unroll
-    for (int w = 1; w < K; w *= 3)
-#pragma unroll
-        for (int k = 0; k + w < K; k += 3 * w)
-            t[k] = (k + 2 * w < K) ? A::vmax3(t[k], t[k + w], t[k + 2 * w]) : A::vmax(t[k], t[k + w]);
-    return t[0];
+    for (int j = 0; j < NOPS; ++j)
+        q[K + j] = (K - 2 * j >= 3) ? A::vmax3(q[3 * j], q[3 * j + 1], q[3 * j + 2]) : A::vmax(q[3 * j], q[3 * j + 1]);
+    return q[K + NOPS - 1];
 }
 
 // Bits 0 .. n-1 set (n clamped to [0, 16]): the columns of a block inside the pair.
