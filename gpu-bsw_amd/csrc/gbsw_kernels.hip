@@ -122,6 +122,12 @@ struct Packed16 {
     // equality with a splat value, on the raw halves
     __device__ static bool eq_lo(V x, V t) { return (uint16_t)x == (uint16_t)t; }
     __device__ static bool eq_hi(V x, V t) { return (uint16_t)(x >> 16) == (uint16_t)(t >> 16); }
+    // Some half of x reaches the target, given x <= target in both halves and t1
+    // = the target's bits minus one in both halves: cell values are >= 0, where
+    // the raw 16-bit patterns (int16 and f16 alike) order as the values do, so
+    // one saturating unsigned subtraction and one compare decide it.
+    __device__ static V target_less1(V t) { return t - 0x00010001u; }
+    __device__ static bool reach_any(V x, V t1) { return raw(__builtin_elementwise_sub_sat(as_u(x), as_u(t1))) != 0u; }
     // 1 in each half equal to t's (raw bits), else 0
     __device__ static uint32_t eq_bits(V x, V t)
     {
@@ -218,6 +224,8 @@ struct Wide32 {
     __device__ static V decay(V X, V G, V Ho) { return vmax(subf0(X, G), Ho); }
     __device__ static bool eq_lo(V x, V t) { return x.l == t.l; }
     __device__ static bool eq_hi(V x, V t) { return x.h == t.h; }
+    __device__ static V target_less1(V t) { return t; }
+    __device__ static bool reach_any(V x, V t) { return x.l == t.l || x.h == t.h; }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
     template <int K>
     __device__ static V prof8(const uint32_t* plo, const uint32_t* phi, int k)
@@ -498,7 +506,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     const bool multi_strip = strips_max > 1;
     const V OPEN2 = A::gap(a.gap_open);
     const V EXT2 = A::gap(a.gap_extend);
-    const V TGT = A::splat(target); // reverse pass: the forward score
+    const V TGT = A::splat(target); // reverse pass: the forward score (the box's maximum)
+    const V TGT1 = A::target_less1(TGT);
     const int v_lo = l, v_hi = l + 16;
     uint64_t cand_lo = 0, cand_hi = 0;
 
@@ -689,7 +698,7 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
     #pragma unroll
                         for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
                     } else {
-                        if (A::eq_lo(rm, TGT) || A::eq_hi(rm, TGT)) {
+                        if (A::reach_any(rm, TGT1)) {
                             const int row_lo = s - v_lo, row_hi = s - v_hi;
                             // first (smallest) column of this row that reaches the target
                             int k_lo = K, k_hi = K;

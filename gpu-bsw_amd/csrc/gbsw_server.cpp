@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -45,6 +47,12 @@ using Clock = std::chrono::steady_clock;
 double ms_since(Clock::time_point a, Clock::time_point b)
 {
     return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+Clock::time_point trace_epoch()
+{
+    static const Clock::time_point t0 = Clock::now();
+    return t0;
 }
 
 constexpr int kMaxBuckets = 64;
@@ -203,7 +211,19 @@ struct Slot {
     bool busy = false;
     uint64_t launch_seq = 0;
     int32_t launch_rc = GBSW_OK;
+    Clock::time_point t_admit, t_launched; // host times of the micro-batch (trace)
 };
+
+// GBSW_SERVER_TRACE=<path>: one line per micro-batch (host admit and launch
+// times, device time, host completion) for diagnosing the serving loop.
+FILE* trace_file()
+{
+    static FILE* f = []() -> FILE* {
+        const char* p = std::getenv("GBSW_SERVER_TRACE");
+        return p ? std::fopen(p, "w") : nullptr;
+    }();
+    return f;
+}
 
 bool same_scoring(const gbsw_scoring& a, const gbsw_scoring& b)
 {
@@ -558,6 +578,7 @@ int gbsw_server::launch(Slot& s)
 // Results of a completed (or failed) micro-batch into its requests.
 void gbsw_server::finish(Slot& s, bool device_ok)
 {
+    const Clock::time_point t_seen = Clock::now();
     int32_t rc = s.launch_rc;
     double busy = 0.0;
     if (rc == GBSW_OK && device_ok) {
@@ -593,6 +614,12 @@ void gbsw_server::finish(Slot& s, bool device_ok)
         }
     }
     const Clock::time_point now = Clock::now();
+    if (FILE* tf = trace_file())
+        std::fprintf(tf, "mb %llu slot %p alphabet %d pairs %lld parts %zu cells %lld admit_launch_ms %.3f "
+                     "device_ms %.3f seen_after_launch_ms %.3f finish_ms %.3f t_ms %.3f\n",
+                     (unsigned long long)s.launch_seq, (void*)&s, (int)s.job.alphabet, (long long)s.job.n,
+                     s.parts.size(), (long long)s.job.cells, ms_since(s.t_admit, s.t_launched), busy,
+                     ms_since(s.t_launched, t_seen), ms_since(t_seen, now), ms_since(trace_epoch(), now));
     std::lock_guard<std::mutex> lk(mu);
     stats.micro_batches += 1;
     stats.busy_ms += busy;
@@ -652,9 +679,11 @@ void gbsw_server::run()
                     break;
                 }
             if (!free_slot) break;
+            free_slot->t_admit = Clock::now();
             admit(*free_slot);
             lk.unlock();
             free_slot->launch_rc = free_slot->parts.empty() ? GBSW_E_INTERNAL : launch(*free_slot);
+            free_slot->t_launched = Clock::now();
             lk.lock();
             progress = true;
         }
