@@ -6,14 +6,16 @@
 //
 // Life of a request:
 //   submit (caller thread): validate, rebase offsets, summarise lengths per
-//     length bucket; copy the residues into the pinned arena unless they already
-//     live in gbsw_host_alloc memory; queue it under its scoring class.
+//     length bucket; upload the residues by DMA into the device staging ring
+//     (from gbsw_host_alloc memory in place, other memory through the pinned
+//     arena); queue it under its scoring class.
 //   admit (scheduler thread): when a slot is free, take the scoring class of
 //     the oldest pending request and pack its queue, oldest first, into the slot
 //     while the slot's pair, residue and strip-boundary capacities hold; a
 //     request that does not fit is split at a pair boundary.
 //   launch: one host->device copy of the offsets and gather tiles, the gather
-//     kernel (pinned host -> HBM, alphabet check), the batched pipeline of
+//     kernel (staging ring -> the slot's packed buffers in HBM, alphabet check),
+//     after the uploads enqueued so far; the batched pipeline of
 //     gbsw_exec.cpp on the slot's streams, one device->host copy of the results.
 //   complete: when the slot's event has fired, results are scattered into each
 //     request's output arrays; a request whose parts are all done is reported.
@@ -94,6 +96,7 @@ struct Arena {
     std::deque<Blk> blocks;
     uint64_t front_seq = 0; // sequence number of blocks.front()
 
+    bool on_device = false;
     hipError_t init(size_t bytes)
     {
         cap = std::max<size_t>(bytes, 4096) & ~(size_t)15;
@@ -107,10 +110,22 @@ struct Arena {
         dev = (const uint8_t*)d;
         return e;
     }
+    // the same ring over device memory (host is null)
+    hipError_t init_device(size_t bytes)
+    {
+        cap = std::max<size_t>(bytes, 4096) & ~(size_t)15;
+        void* d = nullptr;
+        const hipError_t e = hipMalloc(&d, cap);
+        dev = e == hipSuccess ? (const uint8_t*)d : nullptr;
+        on_device = e == hipSuccess;
+        return e;
+    }
     void destroy()
     {
         if (host) (void)hipHostFree(host);
+        if (on_device && dev) (void)hipFree((void*)dev);
         host = nullptr;
+        dev = nullptr;
     }
     // Offset of a new block of `bytes`, or -1 while the space is taken.
     int64_t alloc(size_t bytes, uint64_t* seq)
@@ -182,6 +197,8 @@ struct Request {
     bool done = false, launched = false;
     bool has_arena = false;
     uint64_t arena_seq = 0;
+    bool has_darena = false;   // its residues are in the device staging ring
+    uint64_t darena_seq = 0;
     Clock::time_point t_submit, t_launch, t_done;
     std::list<Request*>::iterator done_it;
 };
@@ -205,6 +222,7 @@ struct Slot {
     PinnedBuffer meta_host, out_host, flags_host;
     size_t sort_bytes = 0;
     hipEvent_t ev_begin = nullptr, ev_done = nullptr;
+    hipEvent_t ev_up = nullptr; // the uploads enqueued before this micro-batch's launch
     std::vector<Part> parts;
     std::vector<GatherTile> tiles;
     int tables = 0;
@@ -238,7 +256,9 @@ struct gbsw_server {
     int device = 0;
     gbsw_server_config cfg;
     AlphabetTables tabs[2];
-    Arena arena;
+    Arena arena;              // pinned: residues of requests given in pageable memory
+    Arena darena;             // device: every request's residues, uploaded at submit
+    hipStream_t up_stream = nullptr; // the uploads (DMA), in submit order
     std::vector<Slot*> slots;
     std::vector<ScoreClass> classes;
     std::unordered_map<uint64_t, Request*> requests;
@@ -262,6 +282,22 @@ struct gbsw_server {
     int launch(Slot& s);               // without mu
     void finish(Slot& s, bool device_ok); // without mu
     void run();
+    // frees a completed request's staging blocks (under mu); true if any
+    bool release_staging(Request* r)
+    {
+        bool any = false;
+        if (r->has_arena) {
+            arena.release(r->arena_seq);
+            r->has_arena = false;
+            any = true;
+        }
+        if (r->has_darena) {
+            darena.release(r->darena_seq);
+            r->has_darena = false;
+            any = true;
+        }
+        return any;
+    }
 };
 
 namespace {
@@ -349,6 +385,7 @@ int gbsw_server::init_slot(Slot& s)
     GBSW_HIP_TRY(s.flags_host.reserve(sizeof(int32_t) * P));
     GBSW_HIP_TRY(hipEventCreate(&s.ev_begin));
     GBSW_HIP_TRY(hipEventCreate(&s.ev_done));
+    GBSW_HIP_TRY(hipEventCreateWithFlags(&s.ev_up, hipEventDisableTiming));
     return GBSW_OK;
 }
 
@@ -487,11 +524,7 @@ void gbsw_server::complete_unlaunched(Request* r)
     if (r->parts_pending > 0) return; // its earlier parts report it
     r->done = true;
     r->t_done = Clock::now();
-    if (r->has_arena) {
-        arena.release(r->arena_seq);
-        r->has_arena = false;
-        cv_space.notify_all();
-    }
+    if (release_staging(r)) cv_space.notify_all();
     stats.completed += 1;
     done_list.push_back(r);
     r->done_it = std::prev(done_list.end());
@@ -549,6 +582,10 @@ int gbsw_server::launch(Slot& s)
     std::memcpy(s.meta_host.as<uint8_t>() + tiles_at, s.tiles.data(), sizeof(GatherTile) * s.tiles.size());
 
     hipStream_t st = s.ex.stream;
+    // the residues of every request of the micro-batch were enqueued on the
+    // upload stream before the request was queued: wait for what it holds now
+    GBSW_HIP_TRY(hipEventRecord(s.ev_up, up_stream));
+    GBSW_HIP_TRY(hipStreamWaitEvent(st, s.ev_up, 0));
     GBSW_HIP_TRY(hipEventRecord(s.ev_begin, st));
     GBSW_HIP_TRY(hipMemcpyAsync(s.meta.ptr, s.meta_host.ptr, meta_bytes, hipMemcpyHostToDevice, st));
     GBSW_HIP_TRY(zero_u32(s.flags.as<uint32_t>(), (int)s.parts.size(), st));
@@ -631,11 +668,7 @@ void gbsw_server::finish(Slot& s, bool device_ok)
         if (r->next == r->n && r->parts_pending == 0 && !r->done) {
             r->done = true;
             r->t_done = now;
-            if (r->has_arena) {
-                arena.release(r->arena_seq);
-                r->has_arena = false;
-                freed = true;
-            }
+            freed = release_staging(r) || freed;
             stats.completed += 1;
             stats.pairs += r->n;
             stats.cells += r->cells;
@@ -725,6 +758,7 @@ void gbsw_server_config_default(gbsw_server_config* cfg)
     cfg->boundary_bytes_per_slot = (int64_t)1 << 30;
     cfg->staging_bytes = (int64_t)256 << 20;
     std::strcpy(cfg->dna_alphabet, "ACGTN");
+    cfg->device_staging_bytes = (int64_t)4 << 30;
 }
 
 int gbsw_server_destroy(gbsw_server* srv);
@@ -743,6 +777,8 @@ int gbsw_server_create(int32_t device, const gbsw_server_config* cfg_in, gbsw_se
     if (cfg.boundary_bytes_per_slot < (int64_t)kPairsPerWave * (int64_t)kMaxLen * 4)
         return fail(GBSW_E_ARGUMENT, "boundary_bytes_per_slot must hold one workgroup at maximal length");
     if (cfg.staging_bytes < 4096) return fail(GBSW_E_ARGUMENT, "staging_bytes must be at least 4096");
+    if (cfg.device_staging_bytes < (int64_t)(2 * kMaxLen) + 64)
+        return fail(GBSW_E_ARGUMENT, "device_staging_bytes must hold one pair of maximal length");
     int count = 0;
     GBSW_HIP_TRY(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return fail(GBSW_E_ARGUMENT, "no such device");
@@ -764,6 +800,12 @@ int gbsw_server_create(int32_t device, const gbsw_server_config* cfg_in, gbsw_se
     if (!rc) {
         const hipError_t e = srv->arena.init((size_t)cfg.staging_bytes);
         if (e != hipSuccess) rc = fail(GBSW_E_MEMORY, std::string("pinned arena: ") + hipGetErrorString(e));
+    }
+    if (!rc) {
+        hipError_t e = srv->darena.init_device((size_t)cfg.device_staging_bytes);
+        if (e == hipSuccess) srv->device_bytes += (int64_t)srv->darena.cap;
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&srv->up_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) rc = fail(GBSW_E_MEMORY, std::string("device staging: ") + hipGetErrorString(e));
     }
     if (rc) {
         const std::string err = gbsw_last_error();
@@ -789,14 +831,20 @@ int gbsw_server_destroy(gbsw_server* srv)
         srv->worker.join();
     }
     (void)hipSetDevice(srv->device);
+    if (srv->up_stream) {
+        (void)hipStreamSynchronize(srv->up_stream);
+        (void)hipStreamDestroy(srv->up_stream);
+    }
     for (auto& kv : srv->requests) delete kv.second;
     for (Slot* s : srv->slots) {
         if (s->ev_begin) (void)hipEventDestroy(s->ev_begin);
+        if (s->ev_up) (void)hipEventDestroy(s->ev_up);
         if (s->ev_done) (void)hipEventDestroy(s->ev_done);
         s->ex.destroy();
         delete s;
     }
     srv->arena.destroy();
+    srv->darena.destroy();
     delete srv;
     return GBSW_OK;
 }
@@ -864,35 +912,69 @@ int gbsw_server_submit(gbsw_server* srv, const gbsw_scoring* sc, int64_t n, cons
             return rc;
         }
     }
+    // Residues go to the device staging ring now, by DMA on the upload stream,
+    // so that they are in HBM while the request waits for a slot (the gather
+    // of its micro-batch then reads HBM).  Pinned sources (gbsw_host_alloc)
+    // are copied from in place; pageable ones go through the pinned arena first.
     const uint8_t* rp = n ? (const uint8_t*)ref + ref_off[0] : nullptr;
     const uint8_t* qp = n ? (const uint8_t*)query + query_off[0] : nullptr;
-    r->ref_src = rbytes ? registered_device_ptr(rp, rbytes) : nullptr;
-    r->qry_src = qbytes ? registered_device_ptr(qp, qbytes) : nullptr;
-    const uint64_t copy_r = (rbytes && !r->ref_src) ? rbytes : 0;
-    const uint64_t copy_q = (qbytes && !r->qry_src) ? qbytes : 0;
-    if (copy_r + copy_q) {
-        const size_t need = (size_t)(copy_r + 16 + copy_q);
+    const bool pin_r = !rbytes || registered_device_ptr(rp, rbytes);
+    const bool pin_q = !qbytes || registered_device_ptr(qp, qbytes);
+    const uint64_t copy_r = pin_r ? 0 : rbytes;
+    const uint64_t copy_q = pin_q ? 0 : qbytes;
+    const uint64_t qdoff = (rbytes + 15) & ~(uint64_t)15; // query bytes in the staging blocks
+    if (rbytes + qbytes) {
+        const size_t dneed = (size_t)(qdoff + qbytes);
+        const size_t pneed = (size_t)(copy_r + 16 + copy_q);
         std::unique_lock<std::mutex> lk(srv->mu);
-        if (((need + 15) & ~(size_t)15) > srv->arena.cap) {
+        if (((dneed + 15) & ~(size_t)15) > srv->darena.cap) {
+            delete r;
+            return fail(GBSW_E_MEMORY, "request larger than the device staging ring (device_staging_bytes)");
+        }
+        if ((copy_r + copy_q) && ((pneed + 15) & ~(size_t)15) > srv->arena.cap) {
             delete r;
             return fail(GBSW_E_MEMORY, "request larger than the staging arena (use gbsw_host_alloc buffers)");
         }
-        int64_t off;
-        uint64_t seq = 0;
-        while ((off = srv->arena.alloc(need, &seq)) < 0) srv->cv_space.wait(lk);
-        r->has_arena = true;
-        r->arena_seq = seq;
+        int64_t doff;
+        uint64_t dseq = 0;
+        while ((doff = srv->darena.alloc(dneed, &dseq)) < 0) srv->cv_space.wait(lk);
+        r->has_darena = true;
+        r->darena_seq = dseq;
+        int64_t poff = 0;
+        if (copy_r + copy_q) {
+            uint64_t pseq = 0;
+            while ((poff = srv->arena.alloc(pneed, &pseq)) < 0) srv->cv_space.wait(lk);
+            r->has_arena = true;
+            r->arena_seq = pseq;
+        }
         lk.unlock();
-        uint8_t* dst = srv->arena.host + off;
-        if (copy_r) {
-            std::memcpy(dst, rp, copy_r);
-            r->ref_src = srv->arena.dev + off;
+        const uint8_t* src_r = rp;
+        const uint8_t* src_q = qp;
+        if (copy_r + copy_q) {
+            uint8_t* dst = srv->arena.host + poff;
+            if (copy_r) {
+                std::memcpy(dst, rp, copy_r);
+                src_r = dst;
+            }
+            if (copy_q) {
+                const size_t qoff = (size_t)((copy_r + 15) & ~(uint64_t)15);
+                std::memcpy(dst + qoff, qp, copy_q);
+                src_q = dst + qoff;
+            }
         }
-        if (copy_q) {
-            const size_t qoff = (size_t)((copy_r + 15) & ~(uint64_t)15);
-            std::memcpy(dst + qoff, qp, copy_q);
-            r->qry_src = srv->arena.dev + off + qoff;
+        uint8_t* dd = (uint8_t*)srv->darena.dev + doff;
+        hipError_t e = hipSuccess;
+        if (rbytes) e = hipMemcpyAsync(dd, src_r, rbytes, hipMemcpyHostToDevice, srv->up_stream);
+        if (e == hipSuccess && qbytes) e = hipMemcpyAsync(dd + qdoff, src_q, qbytes, hipMemcpyHostToDevice, srv->up_stream);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(srv->up_stream); // nothing of this request stays in flight
+            std::lock_guard<std::mutex> lk2(srv->mu);
+            if (srv->release_staging(r)) srv->cv_space.notify_all();
+            delete r;
+            return fail(GBSW_E_DEVICE, std::string("request upload: ") + hipGetErrorString(e));
         }
+        r->ref_src = dd;
+        r->qry_src = dd + qdoff;
     }
     std::lock_guard<std::mutex> lk(srv->mu);
     r->ticket = srv->next_ticket++;

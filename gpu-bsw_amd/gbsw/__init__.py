@@ -77,6 +77,7 @@ class ServerConfig(ctypes.Structure):
         ("boundary_bytes_per_slot", ctypes.c_int64),
         ("staging_bytes", ctypes.c_int64),
         ("dna_alphabet", ctypes.c_char * 32),
+        ("device_staging_bytes", ctypes.c_int64),
     ]
 
 
