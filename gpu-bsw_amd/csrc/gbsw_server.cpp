@@ -6,16 +6,17 @@
 //
 // Life of a request:
 //   submit (caller thread): validate, rebase offsets, summarise lengths per
-//     length bucket; upload the residues by DMA into the device staging ring
-//     (from gbsw_host_alloc memory in place, other memory through the pinned
-//     arena); queue it under its scoring class.
+//     length bucket; copy pageable residues into the pinned arena; upload those
+//     of a large request by DMA into the device staging ring; queue it under
+//     its scoring class.
 //   admit (scheduler thread): when a slot is free, take the scoring class of
 //     the oldest pending request and pack its queue, oldest first, into the slot
 //     while the slot's pair, residue and strip-boundary capacities hold; a
 //     request that does not fit is split at a pair boundary.
 //   launch: one host->device copy of the offsets and gather tiles, the gather
-//     kernel (staging ring -> the slot's packed buffers in HBM, alphabet check),
-//     after the uploads enqueued so far; the batched pipeline of
+//     kernel (staging ring or pinned host memory -> the slot's packed buffers
+//     in HBM, alphabet check), after the uploads enqueued so far; the batched
+//     pipeline of
 //     gbsw_exec.cpp on the slot's streams, one device->host copy of the results.
 //   complete: when the slot's event has fired, results are scattered into each
 //     request's output arrays; a request whose parts are all done is reported.
@@ -62,6 +63,8 @@ constexpr int kMaxBuckets = 64;
 // region rewritten by the host after an earlier read is never served stale.
 constexpr unsigned kDeviceReadFlags = hipHostMallocMapped | hipHostMallocCoherent;
 constexpr uint64_t kMaxLen = 65000;
+// Requests of at least this many residue bytes are uploaded by DMA at submit.
+constexpr uint64_t kDmaMinBytes = (uint64_t)256 << 10;
 
 // ---- pinned host blocks handed out by gbsw_host_alloc ----------------------
 struct HostBlock {
@@ -912,69 +915,85 @@ int gbsw_server_submit(gbsw_server* srv, const gbsw_scoring* sc, int64_t n, cons
             return rc;
         }
     }
-    // Residues go to the device staging ring now, by DMA on the upload stream,
-    // so that they are in HBM while the request waits for a slot (the gather
-    // of its micro-batch then reads HBM).  Pinned sources (gbsw_host_alloc)
-    // are copied from in place; pageable ones go through the pinned arena first.
+    // Residues of a request of at least kDmaMinBytes go to the device staging
+    // ring now, by DMA on the upload stream (one copy when ref and query bytes
+    // are adjacent), so that they are in HBM while the request waits for a slot
+    // and its micro-batch's gather reads HBM.  Smaller requests, for which a
+    // DMA's fixed cost (about 12 us per copy) would dominate, are read by the
+    // gather kernel from pinned host memory.  Pinned sources (gbsw_host_alloc)
+    // are read in place; pageable ones are copied into the pinned arena first.
     const uint8_t* rp = n ? (const uint8_t*)ref + ref_off[0] : nullptr;
     const uint8_t* qp = n ? (const uint8_t*)query + query_off[0] : nullptr;
-    const bool pin_r = !rbytes || registered_device_ptr(rp, rbytes);
-    const bool pin_q = !qbytes || registered_device_ptr(qp, qbytes);
-    const uint64_t copy_r = pin_r ? 0 : rbytes;
-    const uint64_t copy_q = pin_q ? 0 : qbytes;
-    const uint64_t qdoff = (rbytes + 15) & ~(uint64_t)15; // query bytes in the staging blocks
+    const uint8_t* rdev = rbytes ? registered_device_ptr(rp, rbytes) : nullptr;
+    const uint8_t* qdev = qbytes ? registered_device_ptr(qp, qbytes) : nullptr;
+    const uint64_t copy_r = (rbytes && !rdev) ? rbytes : 0;
+    const uint64_t copy_q = (qbytes && !qdev) ? qbytes : 0;
+    const bool dma = rbytes + qbytes >= kDmaMinBytes;
     if (rbytes + qbytes) {
-        const size_t dneed = (size_t)(qdoff + qbytes);
-        const size_t pneed = (size_t)(copy_r + 16 + copy_q);
+        // pinned arena block: the ref bytes, then the query bytes right after them
+        const size_t pneed = (size_t)(copy_r + copy_q);
+        const size_t dneed = (size_t)(rbytes + qbytes);
         std::unique_lock<std::mutex> lk(srv->mu);
-        if (((dneed + 15) & ~(size_t)15) > srv->darena.cap) {
+        if (dma && ((dneed + 15) & ~(size_t)15) > srv->darena.cap) {
             delete r;
             return fail(GBSW_E_MEMORY, "request larger than the device staging ring (device_staging_bytes)");
         }
-        if ((copy_r + copy_q) && ((pneed + 15) & ~(size_t)15) > srv->arena.cap) {
+        if (pneed && ((pneed + 15) & ~(size_t)15) > srv->arena.cap) {
             delete r;
             return fail(GBSW_E_MEMORY, "request larger than the staging arena (use gbsw_host_alloc buffers)");
         }
-        int64_t doff;
-        uint64_t dseq = 0;
-        while ((doff = srv->darena.alloc(dneed, &dseq)) < 0) srv->cv_space.wait(lk);
-        r->has_darena = true;
-        r->darena_seq = dseq;
-        int64_t poff = 0;
-        if (copy_r + copy_q) {
+        int64_t doff = 0, poff = 0;
+        if (dma) {
+            uint64_t dseq = 0;
+            while ((doff = srv->darena.alloc(dneed, &dseq)) < 0) srv->cv_space.wait(lk);
+            r->has_darena = true;
+            r->darena_seq = dseq;
+        }
+        if (pneed) {
             uint64_t pseq = 0;
             while ((poff = srv->arena.alloc(pneed, &pseq)) < 0) srv->cv_space.wait(lk);
             r->has_arena = true;
             r->arena_seq = pseq;
         }
         lk.unlock();
-        const uint8_t* src_r = rp;
-        const uint8_t* src_q = qp;
-        if (copy_r + copy_q) {
-            uint8_t* dst = srv->arena.host + poff;
-            if (copy_r) {
-                std::memcpy(dst, rp, copy_r);
-                src_r = dst;
-            }
-            if (copy_q) {
-                const size_t qoff = (size_t)((copy_r + 15) & ~(uint64_t)15);
-                std::memcpy(dst + qoff, qp, copy_q);
-                src_q = dst + qoff;
-            }
+        // host (and device-mapped) addresses of the bytes to move
+        const uint8_t* hr = rp;
+        const uint8_t* hq = qp;
+        const uint8_t* mr = rdev;
+        const uint8_t* mq = qdev;
+        if (copy_r) {
+            std::memcpy(srv->arena.host + poff, rp, copy_r);
+            hr = srv->arena.host + poff;
+            mr = srv->arena.dev + poff;
         }
-        uint8_t* dd = (uint8_t*)srv->darena.dev + doff;
-        hipError_t e = hipSuccess;
-        if (rbytes) e = hipMemcpyAsync(dd, src_r, rbytes, hipMemcpyHostToDevice, srv->up_stream);
-        if (e == hipSuccess && qbytes) e = hipMemcpyAsync(dd + qdoff, src_q, qbytes, hipMemcpyHostToDevice, srv->up_stream);
-        if (e != hipSuccess) {
-            (void)hipStreamSynchronize(srv->up_stream); // nothing of this request stays in flight
-            std::lock_guard<std::mutex> lk2(srv->mu);
-            if (srv->release_staging(r)) srv->cv_space.notify_all();
-            delete r;
-            return fail(GBSW_E_DEVICE, std::string("request upload: ") + hipGetErrorString(e));
+        if (copy_q) {
+            std::memcpy(srv->arena.host + poff + copy_r, qp, copy_q);
+            hq = srv->arena.host + poff + copy_r;
+            mq = srv->arena.dev + poff + copy_r;
         }
-        r->ref_src = dd;
-        r->qry_src = dd + qdoff;
+        if (!dma) {
+            r->ref_src = mr;
+            r->qry_src = mq;
+        } else {
+            uint8_t* dd = (uint8_t*)srv->darena.dev + doff;
+            hipError_t e = hipSuccess;
+            if (rbytes && qbytes && hq == hr + rbytes) {
+                e = hipMemcpyAsync(dd, hr, rbytes + qbytes, hipMemcpyHostToDevice, srv->up_stream);
+            } else {
+                if (rbytes) e = hipMemcpyAsync(dd, hr, rbytes, hipMemcpyHostToDevice, srv->up_stream);
+                if (e == hipSuccess && qbytes)
+                    e = hipMemcpyAsync(dd + rbytes, hq, qbytes, hipMemcpyHostToDevice, srv->up_stream);
+            }
+            if (e != hipSuccess) {
+                (void)hipStreamSynchronize(srv->up_stream); // nothing of this request stays in flight
+                std::lock_guard<std::mutex> lk2(srv->mu);
+                if (srv->release_staging(r)) srv->cv_space.notify_all();
+                delete r;
+                return fail(GBSW_E_DEVICE, std::string("request upload: ") + hipGetErrorString(e));
+            }
+            r->ref_src = dd;
+            r->qry_src = dd + rbytes;
+        }
     }
     std::lock_guard<std::mutex> lk(srv->mu);
     r->ticket = srv->next_ticket++;

@@ -241,8 +241,9 @@ typedef struct {
     char dna_alphabet[32];           /* DNA bytes accepted, NUL-terminated (default "ACGTN"); other
                                         bytes fail the request with GBSW_E_RESIDUE          */
     int64_t device_staging_bytes;    /* HBM ring holding the residues of submitted, unfinished
-                                        requests, uploaded by DMA at submit (default 4 GiB);
-                                        submit blocks while it is full                       */
+                                        requests of 256 KiB of residues or more, uploaded by DMA
+                                        at submit (default 4 GiB); submit blocks while it is
+                                        full                                                 */
 } gbsw_server_config;
 
 typedef struct {
@@ -270,11 +271,11 @@ int gbsw_server_destroy(gbsw_server* srv);
 /* Queue n pairs (layout as gbsw_batch_create) for alignment under `scoring`.
  * Returns at once with a ticket; the five n-entry int32 result arrays are
  * written before the request is reported complete and must stay valid until
- * then.  The residues are uploaded into the server's device staging ring by
- * DMA: input buffers allocated with gbsw_host_alloc are copied from in place
- * and must stay unchanged until completion; other input buffers are first
- * copied into the server's pinned arena and may be reused when this returns.
- * Thread-safe. */
+ * then.  Input buffers allocated with gbsw_host_alloc are read in place (by
+ * DMA into the server's device staging ring at submit for requests of 256 KiB
+ * of residues or more, by the device over PCIe otherwise) and must stay
+ * unchanged until completion; other input buffers are first copied into the
+ * server's pinned arena and may be reused when this returns.  Thread-safe. */
 int gbsw_server_submit(gbsw_server* srv, const gbsw_scoring* scoring, int64_t n, const char* ref,
                        const uint64_t* ref_off, const char* query, const uint64_t* query_off,
                        int32_t* score, int32_t* ref_begin, int32_t* ref_end, int32_t* query_begin,

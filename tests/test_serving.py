@@ -189,3 +189,39 @@ def test_serving_custom_dna_alphabet(ctx):
     assert srv.wait(t)["status"] == gbsw.GBSW_OK
     check(out, exp, 50, "custom alphabet")
     srv.close()
+
+
+def test_serving_large_requests_uploaded_by_dma(ctx):
+    """Requests of 256 KiB of residues or more are uploaded by DMA into the
+    device staging ring at submit (from gbsw_host_alloc memory at unaligned
+    offsets, and from pageable memory through the pinned arena), small ones
+    read over PCIe by the gather; mixed in one stream, every request equals
+    the oracle, also when the staging ring is small enough to wrap and block."""
+    w = synth.batched_dna(6000, seed=0x5EED0404)
+    exp = expect(ctx, w, w.scoring)
+    sc, _ = scorings(w.alphabet, w.scoring)
+    nr, nq = len(w.ref), len(w.query)
+    hb = gbsw.HostBuffer(nr + nq + 64)
+    a = hb.array
+    r0, q0 = 5, 5 + nr + 7
+    a[r0:r0 + nr] = w.ref
+    a[q0:q0 + nq] = w.query
+    srv = gbsw.Server(0, slots=2, device_staging_bytes=3 << 20, staging_bytes=4 << 20)
+    got = []
+    # request sizes around the threshold (about 0.75 KiB of residues per pair)
+    cuts = [0, 2000, 3500, 3510, 3800, 4500, 4530, 6000]
+    for i, (b, e) in enumerate(zip(cuts[:-1], cuts[1:])):
+        if i % 2 == 0:
+            ro = (w.ref_off[b:e + 1] + np.uint64(r0)).astype(np.uint64)
+            qo = (w.query_off[b:e + 1] + np.uint64(q0)).astype(np.uint64)
+            t, out = srv.submit(sc, a, ro, a, qo)
+        else:
+            sub = w.subset(np.arange(b, e))
+            t, out = srv.submit(sc, sub.ref, sub.ref_off, sub.query, sub.query_off)
+        got.append((t, out, b, e))
+    for t, out, b, e in got:
+        info = srv.wait(t, timeout=60)
+        assert info["status"] == gbsw.GBSW_OK
+        check(out, {f: exp[f][b:e] for f in FIELDS}, e - b, "request [%d, %d)" % (b, e))
+    srv.close()
+    hb.close()
