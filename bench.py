@@ -44,6 +44,7 @@ from gbsw import synth  # noqa: E402
 # cell update (E: add+max, F: add+max, shared H+open, substitution, diagonal add,
 # max3+max0, best tracking cmp+max+select).
 MI355X_CU, MI355X_GHZ, OPS_PER_CELL = 256, 2.4, 12
+DEFAULT_GEOMETRY = 1  # the library's default length -> geometry map (GBSW_KMODEL unset)
 PEAK_INT_TOPS_PACKED = MI355X_CU * 64 * MI355X_GHZ * 2 / 1000.0  # 78.6 TOP/s (int16x2)
 
 
@@ -292,6 +293,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the 10k DNA / 20k protein / long-protein lines")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default length -> geometry map")
+    ap.add_argument("--geometry", type=int, default=-1,
+                    help="use this length -> geometry map (0-5) instead of autotuning one")
     ap.add_argument("--long-pairs", type=int, default=50_000, help="pairs of the long-protein set")
     ap.add_argument("--no-traceback", action="store_true", help="skip the alignment-path (CIGAR) line")
     ap.add_argument("--no-dropin", action="store_true", help="skip the end-to-end drop-in calls")
@@ -326,10 +329,15 @@ def main():
         del src, dst
 
     ctx = gbsw.Context(0)
+    tuned = None
+    if args.geometry >= 0:
+        # (a batch takes its context's map when it is created)
+        ctx.set_option(gbsw.OPT_GEOMETRY, args.geometry)
+        tuned = {"geometry_model": args.geometry, "pinned": True}
+        log("geometry model %d (pinned)" % args.geometry)
     batch = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
     scoring = gbsw.dna_scoring(*w.scoring)
-    tuned = None
-    if not args.no_autotune:
+    if args.geometry < 0 and not args.no_autotune:
         # untimed set-up: the length -> geometry map measured on this batch
         t = time.perf_counter()
         model, model_ms = batch.autotune(scoring, repeats=3)
@@ -397,6 +405,7 @@ def main():
                 "kernel": "%s (%d launch(es) per step, %d pairs, %.4g cells per launch)" % (
                     kname, dom["launches"], dom["pairs"], dom_cells),
                 "kernel_ms": round(dom_ms, 3), "ops_per_cell": OPS_PER_CELL,
+                "geometry_model": tuned["geometry_model"] if tuned else DEFAULT_GEOMETRY,
                 "aggregate_achieved": round(agg, 3), "aggregate_frac": round(agg / PEAK_INT_TOPS_PACKED, 4)}
 
     tb = None

@@ -276,8 +276,12 @@ double occupancy_factor(int K, int alpha, double threshold)
         if (it != cache.end()) {
             wg_per_cu = it->second;
         } else {
+            // one wave each; a small alphabet's usual tables (exact f16 high
+            // bytes) run the f16 kernels on a byte profile
             const bool p8 = byte_profile_alphabet(alpha);
-            wg_per_cu = kernel_occupancy(with_profile(kBatched, p8), K, lds_for(K, alpha, p8)); // one wave each
+            wg_per_cu = kernel_occupancy(with_profile(kBatched, p8), K, lds_for(K, alpha, p8));
+            if (!p8 && kernel_has(kBatchedF16, K))
+                wg_per_cu = std::max(wg_per_cu, kernel_occupancy(with_profile(kBatchedF16, true), K, lds_for(K, alpha, true)));
             cache.emplace(key, wg_per_cu);
         }
     }
@@ -383,14 +387,25 @@ void count_buckets(Job& job, const int32_t* len_ref, const int32_t* len_qry)
 
 int batched_grid_cap(const Exec& ex, int K, int alpha)
 {
-    // the f16 and int16 kernels of a geometry share its LDS layout; take the
-    // larger occupancy, so that a launch's grid (hence its boundary rows) never
-    // exceeds what admission reserved for it
+    // the largest occupancy of the geometry's kernels and profile layouts, so
+    // that a launch's grid (hence its boundary rows) never exceeds what
+    // admission reserved for it
     const bool p8 = byte_profile_alphabet(alpha);
-    const int lds = lds_for(K, alpha, p8);
-    int occ = kernel_occupancy(with_profile(kBatched, p8), K, lds);
-    if (!p8 && kernel_has(kBatchedF16, K)) occ = std::max(occ, kernel_occupancy(kBatchedF16, K, lds));
+    int occ = kernel_occupancy(with_profile(kBatched, p8), K, lds_for(K, alpha, p8));
+    if (kernel_has(kBatchedF16, K)) {
+        occ = std::max(occ, kernel_occupancy(kBatchedF16, K, lds_for(K, alpha, false)));
+        occ = std::max(occ, kernel_occupancy(with_profile(kBatchedF16, true), K, lds_for(K, alpha, true)));
+    }
     return std::max(1, occ) * ex.num_cu;
+}
+
+bool f16_high_byte_exact(int32_t s)
+{
+    // an integer is an f16 value with a zero low byte when it has at most three
+    // significant bits (leading one and two mantissa bits) below 2^11
+    const int32_t a = s < 0 ? -s : s;
+    if (a >= kF16Exact) return false;
+    return a == 0 || (a >> __builtin_ctz((uint32_t)a)) < 8;
 }
 
 int batched_kind(const Bucket& bk, int32_t max_sub, bool allow_f16)
@@ -422,7 +437,8 @@ bool table_is_asymmetric(const gbsw_scoring* sc, const Job& job)
     return false;
 }
 
-int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8)
+int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8,
+                    bool* f16p8)
 {
     if (sc->alphabet != job.alphabet) return fail(GBSW_E_ARGUMENT, "scoring alphabet differs from the batch alphabet");
     if (sc->gap_open > 0 || sc->gap_extend > 0) return fail(GBSW_E_SCORING, "gap_open and gap_extend must be <= 0");
@@ -450,6 +466,11 @@ int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>
     int lo = 0;
     for (int16_t v : sub) lo = std::min<int>(lo, v);
     *p8 = byte_profile_alphabet(A) && lo >= -127 && mx <= 127;
+    if (f16p8) {
+        // the f16 kernels can read a byte profile (the scores' f16 high bytes)
+        *f16p8 = true;
+        for (int16_t v : sub) *f16p8 = *f16p8 && f16_high_byte_exact(v);
+    }
     // result keys hold scores in 23 bits; int16 saturation is handled per pair
     if ((int64_t)job.max_cols * *max_sub >= (1 << 23)) return fail(GBSW_E_RANGE, "possible score exceeds 2^23");
     return GBSW_OK;
@@ -490,22 +511,33 @@ size_t split_words_of(const Bucket& bk, const BucketPlan& b)
 }
 
 // Kernels, grids and scratch of every bucket (host only; reserves the buffers).
-int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, bool asym, int kclamp, int64_t bnd_rows,
+int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, bool f16p8, bool asym, int kclamp, int64_t bnd_rows,
                  std::vector<BucketPlan>& plan, std::vector<size_t>& order)
 {
     const size_t nb = job.buckets.size();
     plan.assign(nb, BucketPlan{});
     order.resize(nb);
+    // the unsplit kernel of a bucket and its LDS layout: the int16 kernels use
+    // the job's profile (p8), the f16 ones a byte profile when the table allows
+    // it (f16p8, a third of the LDS of DNA's int16 profile: more workgroups per
+    // CU) and else the int16 profile, which a byte-profile alphabet does not get
+    const bool allow_f16 = ex.f16 && (f16p8 || !p8);
+    auto unsplit = [&](BucketPlan& b, const Bucket& bk) {
+        const int k = batched_kind(bk, max_sub, allow_f16);
+        const bool bp8 = k == kBatchedF16 ? f16p8 : p8;
+        b.kind = with_profile(k, bp8);
+        b.lds = lds_for(b.K, job.alpha, bp8);
+    };
     for (size_t i = 0; i < nb; ++i) {
         order[i] = i;
         const Bucket& bk = job.buckets[i];
         BucketPlan& b = plan[i];
         b.K = std::min(kclamp, bk.K);
-        b.kind = with_profile(batched_kind(bk, max_sub, ex.f16 && !p8), p8);
-        b.lds = lds_for(b.K, job.alpha, p8);
+        unsplit(b, bk);
+        const bool bp8 = (b.kind & kByteProfile) != 0;
         const int64_t nquads = (bk.count + kPairsPerWave - 1) / kPairsPerWave;
         b.strips = std::max(1, (bk.max_cols + kVirtPerPair * b.K - 1) / (kVirtPerPair * b.K));
-        const int skind = with_profile((b.kind & ~kByteProfile) == kBatchedF16 ? kSplitF16 : kSplit, p8);
+        const int skind = with_profile((b.kind & ~kByteProfile) == kBatchedF16 ? kSplitF16 : kSplit, bp8);
         // A split item waits about four chunks for its predecessor's first
         // rows: worth it only when the bucket has too few quads to keep every
         // workgroup busy for two rounds (on the 200k-pair DNA mix: 1765 vs 827
@@ -554,7 +586,7 @@ int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, bool asym, int 
             for (size_t i = 0; i < nb; ++i)
                 if (plan_is_split(plan[i]) && (w == nb || plan[i].grid > plan[w].grid)) w = i;
             if (w == nb) break;
-            plan[w].kind = with_profile(batched_kind(job.buckets[w], max_sub, ex.f16 && !p8), p8);
+            unsplit(plan[w], job.buckets[w]);
             plan[w].grid = grid_for(ex, plan[w].kind, plan[w].K, plan[w].lds,
                                     (job.buckets[w].count + kPairsPerWave - 1) / kPairsPerWave);
             sizes();
@@ -692,8 +724,8 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     ex.graph_state = 0;
     std::vector<int16_t> sub;
     int max_sub = 0;
-    bool p8 = false;
-    int rc = prepare_scoring(job, sc, sub, &max_sub, &p8);
+    bool p8 = false, f16p8 = false;
+    int rc = prepare_scoring(job, sc, sub, &max_sub, &p8, &f16p8);
     if (rc) return rc;
     ex.max_sub = max_sub;
     ex.p8 = p8;
@@ -761,7 +793,7 @@ int enqueue_job(Exec& ex, const Job& job, const gbsw_scoring* sc, int mode)
     }
     std::vector<BucketPlan> plan;
     std::vector<size_t> order;
-    rc = plan_batched(ex, job, max_sub, p8, a.asym != 0, kclamp, a.bnd_rows, plan, order);
+    rc = plan_batched(ex, job, max_sub, p8, f16p8, a.asym != 0, kclamp, a.bnd_rows, plan, order);
     if (rc) return rc;
     const size_t sub_bytes = sizeof(int16_t) * sub.size();
     if (!ex.graphs || ex.fixed || !job.graph) {

@@ -200,7 +200,8 @@ int sort_scratch_bytes(int64_t n, size_t* bytes);
 bool table_is_asymmetric(const gbsw_scoring* sc, const Job& job);
 // Validate a scoring for a job and build its substitution table; *p8: the job
 // can use the int8 profile.
-int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8);
+int prepare_scoring(const Job& job, const gbsw_scoring* sc, std::vector<int16_t>& sub, int* max_sub, bool* p8,
+                    bool* f16p8 = nullptr);
 
 // Allocation-free entry point (gbsw_align_noalloc): the device scratch is one
 // static array per device (gbsw_noalloc.hip) of kNoallocScratchBytes.

@@ -134,11 +134,13 @@ __host__ __device__ constexpr int32_t kstride(int32_t K) { return K / 8 * 8 + kr
 // the (alpha+1)-row profile of a strip takes half the LDS and leaves room for
 // more waves per CU.  Layout of one row: K/16 blocks of 16 columns for all 32
 // virtual lanes (one b128 read each), then the K%16 remaining columns padded to
-// a b32/b64/b128 read (kremstride8).
+// a b32/b64/b96/b128 read (kremstride8).  With the f16 kernels (DNA-like
+// tables whose scores are f16 values with a zero low byte) a byte is the high
+// byte of the score's f16 bits instead.
 constexpr int kByteProfileAlpha = 8;
 __host__ __device__ constexpr int32_t kremstride8(int32_t K)
 {
-    return K % 16 == 0 ? 0 : K % 16 <= 4 ? 4 : K % 16 <= 8 ? 8 : 16;
+    return K % 16 == 0 ? 0 : K % 16 <= 4 ? 4 : K % 16 <= 8 ? 8 : K % 16 <= 12 ? 12 : 16;
 }
 __host__ __device__ constexpr int32_t kstride8(int32_t K) { return K / 16 * 16 + kremstride8(K); }
 

@@ -135,6 +135,8 @@ struct Packed16 {
         return raw(one - __builtin_elementwise_min(as_u(x ^ t), one));
     }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
+    // byte-profile entry of score s (kNeg, the padding, becomes -128)
+    __device__ static uint8_t prof_byte(int32_t s) { return (uint8_t)(int8_t)max(s, -128); }
     // column k of a byte profile: byte k%4 of word k/4, sign-extended into each
     // half.  v_perm's selectors 8..11 copy the sign bits of bytes 1, 3, 5, 7, so
     // odd bytes take one instruction; even ones go to the high byte of each half
@@ -179,6 +181,16 @@ struct PackedF16 : Packed16 {
     __device__ static int32_t lo(V x) { return f16_int(x & 0xFFFFu); }
     __device__ static int32_t hi(V x) { return f16_int(x >> 16); }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)f16_bits(s); }
+    // Byte profile of a table whose scores (and kNeg) are f16 values with a zero
+    // low byte: the high byte, and column k of it is that byte moved into the
+    // high byte of each half (one v_perm: selector 0x0C is a zero byte).
+    __device__ static uint8_t prof_byte(int32_t s) { return (uint8_t)(f16_bits(s) >> 8); }
+    template <int K>
+    __device__ static V prof8(const uint32_t* plo, const uint32_t* phi, int k)
+    {
+        const uint32_t b = (uint32_t)(k & 3);
+        return __builtin_amdgcn_perm(phi[k / 4], plo[k / 4], 0x000C000Cu | (b << 8) | ((4u + b) << 24));
+    }
 };
 
 // Wide32: the same two virtual lanes as two int32 VGPRs, for pairs whose score
@@ -227,6 +239,7 @@ struct Wide32 {
     __device__ static V target_less1(V t) { return t; }
     __device__ static bool reach_any(V x, V t) { return x.l == t.l || x.h == t.h; }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
+    __device__ static uint8_t prof_byte(int32_t s) { return (uint8_t)(int8_t)max(s, -128); }
     template <int K>
     __device__ static V prof8(const uint32_t* plo, const uint32_t* phi, int k)
     {
@@ -332,6 +345,9 @@ __device__ __forceinline__ void prof_load8(const uint8_t* row, int v, uint32_t (
     if constexpr (RS == 16) {
         const uint4 x = *(const uint4*)r;
         d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y; d[4 * Q + 2] = x.z; d[4 * Q + 3] = x.w;
+    } else if constexpr (RS == 12) {
+        const uint3 x = *(const uint3*)r;
+        d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y; d[4 * Q + 2] = x.z;
     } else if constexpr (RS == 8) {
         const uint2 x = *(const uint2*)r;
         d[4 * Q + 0] = x.x; d[4 * Q + 1] = x.y;
@@ -564,8 +580,8 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                     uint32_t w = 0;
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
-                        const int sc = (code[b] >= 0 && r < alpha) ? sub[r * alpha + code[b]] : -128;
-                        w |= (uint32_t)(uint8_t)sc << (8 * b);
+                        const int sc = (code[b] >= 0 && r < alpha) ? sub[r * alpha + code[b]] : kNeg;
+                        w |= (uint32_t)A::prof_byte(sc) << (8 * b);
                     }
                     *(uint32_t*)(prof8 + r * rowstride + j4 * 4) = w;
                 }
@@ -1325,15 +1341,14 @@ template <int K, bool P8>
 static const void* split_ptr(bool f16)
 {
     if (!f16) return (const void*)&k_split<Packed16, K, P8>;
-    if constexpr (P8) return nullptr;
-    else return (const void*)&k_split<PackedF16, K, false>;
+    return (const void*)&k_split<PackedF16, K, P8>;
 }
 
+// (with P8, the byte profile holds the high bytes of f16 scores)
 template <int K, bool P8>
 static const void* f16_ptr()
 {
-    if constexpr (P8) return nullptr; // f16 kernels read int16 profiles only
-    else return (const void*)&k_batched<PackedF16, K, false>;
+    return (const void*)&k_batched<PackedF16, K, P8>;
 }
 
 // Compiled geometries (a build may narrow the lists with -D for inspection).

@@ -444,3 +444,36 @@ def test_autotune_then_replay_uses_the_kept_map(ctx):
     finally:
         b.close()
         ctx.set_option(gbsw.OPT_GEOMETRY, 1)
+
+
+def test_f16_kernels_take_a_byte_profile_when_the_scores_allow(ctx):
+    """The f16 kernels read a byte profile (the scores' f16 high bytes) when
+    every score has at most three significant bits (3/-3, or BLOSUM62 with 9
+    and 11 moved to 8 and 12), and the int16 profile otherwise (9/-11): all
+    three give the oracle's results."""
+    w = synth.batched_dna(3000, seed=0x5EED020C)
+    f16_kinds = (gbsw.KIND_BATCHED_F16, gbsw.KIND_SPLIT_F16)
+    b = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
+    try:
+        for scoring, byte_profile in (((3, -3, -6, -1), True), ((9, -11, -13, -2), False)):
+            print("case f16 profile", scoring, flush=True)
+            exp = oracle_results(w.ref, w.ref_off, w.query, w.query_off, gbsw.DNA, scoring)
+            b.align(gbsw.dna_scoring(*scoring), "batched")
+            assert_same(b.results(), exp, "dna %s" % (scoring,))
+            f16 = [L for L in b.launches() if L["kind"] & 0xFF in f16_kinds]
+            assert f16, b.launches()
+            assert all(bool(L["kind"] & gbsw.KIND_BYTE_PROFILE) == byte_profile for L in f16), f16
+    finally:
+        b.close()
+    wp = synth.plain_protein(600, seed=0x5EED020D)
+    table = [8 if v == 9 else 12 if v == 11 else v for v in BLOSUM62]
+    exp = ol.align_batch(wp.ref, wp.ref_off, wp.query, wp.query_off, ol.aa_scoring(table, -6, -1))
+    b = ctx.batch(wp.alphabet, wp.ref, wp.ref_off, wp.query, wp.query_off)
+    try:
+        print("case f16 profile protein", flush=True)
+        b.align(gbsw.protein_scoring(table, -6, -1), "batched")
+        assert_same(b.results(), {f: exp[f] for f in FIELDS}, "protein, exact f16 bytes")
+        kinds = {L["kind"] for L in b.launches()}
+        assert kinds & {gbsw.KIND_BATCHED_F16 | gbsw.KIND_BYTE_PROFILE, gbsw.KIND_SPLIT_F16 | gbsw.KIND_BYTE_PROFILE}, kinds
+    finally:
+        b.close()
