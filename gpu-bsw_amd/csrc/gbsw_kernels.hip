@@ -128,6 +128,8 @@ struct Packed16 {
     // one saturating unsigned subtraction and one compare decide it.
     __device__ static V target_less1(V t) { return t - 0x00010001u; }
     __device__ static bool reach_any(V x, V t1) { return raw(__builtin_elementwise_sub_sat(as_u(x), as_u(t1))) != 0u; }
+    // some half differs (raw bits; cell values order as their bits)
+    __device__ static bool differs(V a, V b) { return (a ^ b) != 0u; }
     // 1 in each half equal to t's (raw bits), else 0
     __device__ static uint32_t eq_bits(V x, V t)
     {
@@ -238,6 +240,7 @@ struct Wide32 {
     __device__ static bool eq_hi(V x, V t) { return x.h == t.h; }
     __device__ static V target_less1(V t) { return t; }
     __device__ static bool reach_any(V x, V t) { return x.l == t.l || x.h == t.h; }
+    __device__ static bool differs(V a, V b) { return a.l != b.l || a.h != b.h; }
     __device__ static int16_t prof_val(int32_t s) { return (int16_t)s; }
     __device__ static uint8_t prof_byte(int32_t s) { return (uint8_t)(int8_t)max(s, -128); }
     template <int K>
@@ -706,14 +709,25 @@ __device__ PassResult run_pass(const KernelArgs& a, const Smem& sm, const Task& 
                         if (write_bnd && l == 15) ring_bnd[(s - v_hi) & (kRing - 1)] = A::bnd_out(Hout, Eout);
                     }
 
-                    const V rm = tree_max<A, K>(Hcol);
                     if constexpr (!REV) {
-                        const V m = A::ltmask(best, rm);
-                        best = A::vmax(best, rm);
-                        bstep = A::sel_step(m, s, bstep);
+                        // the new best over the lane's columns and its old best
+                        // (one maximum tree); the snapshot of the step's columns is
+                        // taken in the steps where some lane of the wave improves
+                        // (about two in three on the DNA mix), skipped in the others
+                        V q[K + 1];
     #pragma unroll
-                        for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
+                        for (int k = 0; k < K; ++k) q[k] = Hcol[k];
+                        q[K] = best;
+                        const V nb = tree_max<A, K + 1>(q);
+                        if (__ballot(A::differs(nb, best)) != 0) {
+                            const V m = A::ltmask(best, nb);
+                            bstep = A::sel_step(m, s, bstep);
+    #pragma unroll
+                            for (int k = 0; k < K; ++k) snap[k] = A::sel(m, Hcol[k], snap[k]);
+                        }
+                        best = nb;
                     } else {
+                        const V rm = tree_max<A, K>(Hcol);
                         if (A::reach_any(rm, TGT1)) {
                             const int row_lo = s - v_lo, row_hi = s - v_hi;
                             // first (smallest) column of this row that reaches the target
