@@ -195,15 +195,20 @@ def time_mode(batch, scoring, mode, warmup, steps, cells):
             "launches": avg, "buckets": st["buckets"]}
 
 
-def config_bench(args):
+def config_bench(args, only=None):
     """The other workloads: 10k plain DNA and 20k protein (plain and batched
-    forms, bit-exact between them) and the 50k long-protein set (batched)."""
+    forms, bit-exact between them) and the 50k long-protein set (batched); with
+    `only`, that one workload's batched form (a profiling run)."""
     from gbsw.scoring import BLOSUM62
     out = {}
     ctx = gbsw.Context(0)
     for name, make, plain in (("plain_dna_10k", lambda: synth.plain_dna(), True),
                               ("plain_protein_20k", lambda: synth.plain_protein(), True),
                               ("long_protein", lambda: synth.long_protein(args.long_pairs), False)):
+        if only is not None:
+            if name != only:
+                continue
+            plain = False
         t = time.perf_counter()
         w = make()
         cells = w.cells()
@@ -257,8 +262,12 @@ def serving_bench(args):
 
     srv = gbsw.Server(0, slots=args.serving_slots)
     loadgen.saturate(srv, pool, repeat=1)  # warm-up
-    peak, infos = loadgen.saturate(srv, pool, repeat=2)
-    log("serving peak (whole pool x2 at once): %.1f GCUPS" % peak)
+    burst, _ = loadgen.saturate(srv, pool, repeat=2)
+    # peak: saturation in closed loop (two pools' worth of requests in flight,
+    # each completion replaced at once), the rate the offered loads are set from
+    peak, peak_requests = loadgen.closed_loop(srv, pool, args.serving_peak_window, depth=2 * len(pool.requests))
+    log("serving peak (closed loop, %d requests in flight, %.0f s): %.1f GCUPS (%d requests); whole pool x2 "
+        "at once: %.1f GCUPS" % (2 * len(pool.requests), args.serving_peak_window, peak, peak_requests, burst))
     loads = []
     for frac in [float(x) for x in args.serving_loads.split(",")]:
         rate = frac * peak * 1e9 / pool.mean_cells
@@ -273,7 +282,9 @@ def serving_bench(args):
     st = srv.stats()
     srv.close()
     pool.close()
-    return {"peak_gcups": round(peak, 1), "window_s": args.serving_window, "slots": args.serving_slots,
+    return {"peak_gcups": round(peak, 1), "peak_method": "closed loop, %d requests in flight, %.0f s" % (
+                2 * len(pool.requests), args.serving_peak_window),
+            "burst_gcups": round(burst, 1), "window_s": args.serving_window, "slots": args.serving_slots,
             "pool_requests": len(pool.requests), "loads": loads, "micro_batches": st["micro_batches"],
             "requests": st["completed"], "hbm_bytes": st["device_bytes"],
             "bit_exact_vs_alone": mismatches[0] == 0, "mismatches": mismatches[0]}
@@ -304,7 +315,10 @@ def main():
     ap.add_argument("--serving-window", type=float, default=60.0, help="seconds per offered load")
     ap.add_argument("--serving-loads", default="0.25,0.5,0.75,1.0")
     ap.add_argument("--serving-requests", type=int, default=512, help="request pool size")
-    ap.add_argument("--serving-slots", type=int, default=2)
+    ap.add_argument("--serving-slots", type=int, default=3)
+    ap.add_argument("--serving-peak-window", type=float, default=3.0, help="seconds of the closed-loop peak")
+    ap.add_argument("--only", choices=["plain_dna_10k", "plain_protein_20k", "long_protein"],
+                    help="time only this workload's batched form and print its figures (a profiling run)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (rocprof FETCH/WRITE calibration)")
     args = ap.parse_args()
@@ -314,6 +328,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.only:
+        print(json.dumps({"only": args.only, **config_bench(args, only=args.only)[args.only]}))
+        return
     t = time.perf_counter()
     w = synth.batched_dna(args.pairs)
     cells = w.cells()
@@ -366,10 +383,9 @@ def main():
 
     # Dominant kernel.  The timed steps run the length buckets on four streams, so
     # their launches share the GPU and a launch's duration is not its own.  One
-    # more step runs the buckets one after another: the launch with the longest
-    # duration there is the dominant kernel (one length bucket, persistent, forward
-    # and reverse fused); its algorithmic work = 12 integer ops x the bucket's
-    # forward cells (|ref| x |query|).
+    # more step runs the buckets one after another and times each launch (one
+    # length bucket, persistent, forward and reverse fused); a launch's
+    # algorithmic work = 12 integer ops x the bucket's forward cells (|ref| x |query|).
     # (issued call by call: per-launch durations come from hipEvents around
     # each launch, which a graph replay does not time)
     ctx.set_option(gbsw.OPT_STREAMS, 1)
@@ -377,10 +393,13 @@ def main():
     serial = time_mode(batch, scoring, "batched", 1, 1, cells)
     ctx.set_option(gbsw.OPT_STREAMS, 4)
     ctx.set_option(gbsw.OPT_GRAPHS, 1)
-    # The dominant kernel is the kernel (template instance) with the most time in
-    # the serialised step; one kernel may serve several buckets (a single-strip
-    # and a multi-strip one), so its duration is the average over its launches,
-    # as rocprofv3's kernel statistics give it.
+    # The dominant kernel is the kernel (template instance) that aligns the most
+    # forward cells of the step (fixed by the workload and the length -> geometry
+    # map; the launch durations of the two largest buckets lie within a few per
+    # cent of each other, so choosing by time would flip between them from run to
+    # run).  One kernel may serve several buckets (a single-strip and a
+    # multi-strip one), so its duration is the average over its launches, as
+    # rocprofv3's kernel statistics give it.
     per_kernel = {}
     for L in serial["launches"]:
         kfn, kpol = KIND_NAMES.get(L["kind"] & 0xFF, ("?", "?"))
@@ -391,7 +410,7 @@ def main():
         d["cells"] += L["cells"]
         d["pairs"] += L["pairs"]
         d["launches"] += 1
-    kname, dom = max(per_kernel.items(), key=lambda kv: kv[1]["ms"])
+    kname, dom = max(per_kernel.items(), key=lambda kv: kv[1]["cells"])
     dom_ms = dom["ms"] / dom["launches"]
     dom_cells = dom["cells"] / dom["launches"]
     achieved = OPS_PER_CELL * dom_cells / (dom_ms / 1000.0) / 1e12

@@ -3,7 +3,8 @@
 The serving workload: Poisson arrivals of requests of 1..4096 pairs
 (log-uniform), 70% DNA drawn like the 1M-pair DNA mix and 30% protein drawn like
 the 20,000-pair protein set; offered load is a fraction of the peak GCUPS the
-same server sustains when the whole pool is submitted at once.
+same server sustains at saturation (closed loop: a fixed number of requests in
+flight, each completion replaced at once).
 
 A pool of requests is generated once and laid out in pinned host memory
 (gbsw_host_alloc), so submissions are zero-copy and cost the generator only the
@@ -111,6 +112,55 @@ def saturate(server, pool, repeat=1):
             infos.append(info)
     dt = time.perf_counter() - t0
     return cells / dt / 1e9, infos
+
+
+def closed_loop(server, pool, duration, depth=None, warmup=0.5, seed=0):
+    """Sustained throughput at saturation: `depth` requests (default: two pools'
+    worth) stay in flight, each completion is replaced by the next pool entry
+    (seeded random order), and the GCUPS is the cells of the requests that
+    complete within `duration` seconds after a `warmup`, over that window.
+    Returns (gcups, completed requests in the window)."""
+    rng = np.random.default_rng(seed)
+    depth = depth or 2 * len(pool.requests)
+    outputs = _Outputs(pool.max_pairs)
+    submit = _submit_fn(server, pool, outputs)
+    order = rng.permutation(len(pool.requests))
+    nxt = 0
+    inflight = {}
+
+    def refill():
+        nonlocal nxt
+        while len(inflight) < depth:
+            req = pool.requests[order[nxt % len(order)]]
+            t, out = submit(req)
+            inflight[t] = (out, req["cells"])
+            nxt += 1
+
+    refill()
+    t0 = time.perf_counter()
+    t_on, t_off = t0 + warmup, t0 + warmup + duration
+    cells = 0
+    count = 0
+    while True:
+        now = time.perf_counter()
+        if now >= t_off:
+            break
+        done = server.poll()
+        if not done:
+            time.sleep(0.0002)
+            continue
+        for info in done:
+            out, c = inflight.pop(info["ticket"])
+            outputs.put(out)
+            if t_on <= now < t_off:
+                cells += c
+                count += 1
+        refill()
+    while inflight:  # drain
+        for info in server.poll():
+            outputs.put(inflight.pop(info["ticket"])[0])
+        time.sleep(0.0002)
+    return cells / duration / 1e9, count
 
 
 def poisson(server, pool, rate, duration, seed=0, verify=None):
