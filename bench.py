@@ -260,14 +260,15 @@ def serving_bench(args):
         if info["status"] != 0 or not np.array_equal(out[:, :n], alone[k]):
             mismatches[0] += 1
 
-    srv = gbsw.Server(0, slots=args.serving_slots)
+    srv = gbsw.Server(0, slots=args.serving_slots, pairs_per_slot=args.serving_pairs_per_slot)
     loadgen.saturate(srv, pool, repeat=1)  # warm-up
     burst, _ = loadgen.saturate(srv, pool, repeat=2)
-    # peak: saturation in closed loop (two pools' worth of requests in flight,
+    # peak: saturation in closed loop (four pools' worth of requests in flight,
     # each completion replaced at once), the rate the offered loads are set from
-    peak, peak_requests = loadgen.closed_loop(srv, pool, args.serving_peak_window, depth=2 * len(pool.requests))
+    depth = 4 * len(pool.requests)
+    peak, peak_requests = loadgen.closed_loop(srv, pool, args.serving_peak_window, depth=depth)
     log("serving peak (closed loop, %d requests in flight, %.0f s): %.1f GCUPS (%d requests); whole pool x2 "
-        "at once: %.1f GCUPS" % (2 * len(pool.requests), args.serving_peak_window, peak, peak_requests, burst))
+        "at once: %.1f GCUPS" % (depth, args.serving_peak_window, peak, peak_requests, burst))
     loads = []
     for frac in [float(x) for x in args.serving_loads.split(",")]:
         rate = frac * peak * 1e9 / pool.mean_cells
@@ -283,8 +284,9 @@ def serving_bench(args):
     srv.close()
     pool.close()
     return {"peak_gcups": round(peak, 1), "peak_method": "closed loop, %d requests in flight, %.0f s" % (
-                2 * len(pool.requests), args.serving_peak_window),
+                depth, args.serving_peak_window),
             "burst_gcups": round(burst, 1), "window_s": args.serving_window, "slots": args.serving_slots,
+            "pairs_per_slot": args.serving_pairs_per_slot,
             "pool_requests": len(pool.requests), "loads": loads, "micro_batches": st["micro_batches"],
             "requests": st["completed"], "hbm_bytes": st["device_bytes"],
             "bit_exact_vs_alone": mismatches[0] == 0, "mismatches": mismatches[0]}
@@ -315,7 +317,8 @@ def main():
     ap.add_argument("--serving-window", type=float, default=60.0, help="seconds per offered load")
     ap.add_argument("--serving-loads", default="0.25,0.5,0.75,1.0")
     ap.add_argument("--serving-requests", type=int, default=512, help="request pool size")
-    ap.add_argument("--serving-slots", type=int, default=3)
+    ap.add_argument("--serving-slots", type=int, default=2)
+    ap.add_argument("--serving-pairs-per-slot", type=int, default=524288)
     ap.add_argument("--serving-peak-window", type=float, default=3.0, help="seconds of the closed-loop peak")
     ap.add_argument("--only", choices=["plain_dna_10k", "plain_protein_20k", "long_protein"],
                     help="time only this workload's batched form and print its figures (a profiling run)")
