@@ -490,8 +490,12 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     int64_t* dir_off = order + n;
     int32_t* nruns = (int32_t*)(dir_off + n);
     std::vector<int64_t> region((size_t)n);
-    std::vector<uint8_t> cls((size_t)n);
-    int64_t cls_count[64] = {0};
+    // class of a pair: its stripe count (capped at 127), then its rows in steps of 8
+    // below 2048 and of 128 above (capped): the four pairs of a DP wave run the
+    // largest stripe and row counts of the four, so like pairs go together
+    constexpr int kTbClasses = 128 * 512;
+    std::vector<uint16_t> cls((size_t)n);
+    std::vector<int64_t> cls_count(kTbClasses, 0);
     cap_off[0] = 0;
     for (int64_t p = 0; p < n; ++p) {
         const bool on = res[(size_t)p] > 0;
@@ -501,19 +505,18 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
         if ((int64_t)(m + w) * big >= ((int64_t)1 << 28)) return fail(GBSW_E_RANGE, "path scores may exceed the int32 cells' range");
         cap_off[(size_t)p + 1] = cap_off[(size_t)p] + m + w;
         region[(size_t)p] = tb_region_bytes(m, w);
-        // cost class: steps of the pair's DP, stripes x (rows + 63), in powers of two
-        const int64_t steps = on ? (int64_t)((w + 63) / 64) * (m + 63) : 0;
-        int c = 0;
-        while (c < 63 && ((int64_t)1 << c) < steps) ++c;
-        cls[(size_t)p] = (uint8_t)c;
-        ++cls_count[c];
+        const int32_t stripes = on ? std::min(127, (w + kTbStripeColumns - 1) / kTbStripeColumns) : 0;
+        const int32_t rows = !on ? 0 : m < 2048 ? m >> 3 : std::min(511, 256 + ((m - 2048) >> 7));
+        const int c = stripes * 512 + rows;
+        cls[(size_t)p] = (uint16_t)c;
+        ++cls_count[(size_t)c];
     }
-    // order: longest pairs first (their DP lasts longest; the short ones fill in
-    // behind them), by a counting sort over the cost classes
+    // order: largest classes first (their DP lasts longest; the short ones fill in
+    // behind them), by a counting sort over the classes
     {
-        int64_t start[64];
+        std::vector<int64_t> start(kTbClasses);
         int64_t acc = 0;
-        for (int c = 63; c >= 0; --c) { start[c] = acc; acc += cls_count[c]; }
+        for (int c = kTbClasses - 1; c >= 0; --c) { start[(size_t)c] = acc; acc += cls_count[(size_t)c]; }
         for (int64_t p = 0; p < n; ++p) order[(size_t)start[cls[(size_t)p]]++] = p;
     }
     // launches of at most kTbScratch bytes of scratch (or one pair), in that order

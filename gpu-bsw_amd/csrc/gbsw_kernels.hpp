@@ -76,6 +76,8 @@ struct TbArgs {
     int32_t* nruns;         // per pair: runs written
     int32_t* path_score;    // per pair: score of the path
 };
+// Columns of one stripe of the traceback DP.
+constexpr int kTbStripeColumns = 64;
 // Scratch bytes of a pair with an m x n rectangle (0 when empty).
 int64_t tb_region_bytes(int32_t m, int32_t n);
 // Tightness bytes of the pairs at positions p0 .. p0 + count - 1 of `order`.

@@ -10,13 +10,16 @@
 // op sequence, read from its end, is lexicographically smallest with M < D < I.
 //
 // Three kernels per call:
-//  k_tb_dp:      one wave per pair (four per workgroup).  Lane l owns column 64 s + l + 1 of stripe s
-//                and walks the rectangle's rows along anti-diagonals (row
-//                t - l + 1 at step t); the left neighbour's H and E arrive through
-//                a one-lane DPP shift, the stripe's left boundary column through
-//                memory.  Each cell stores one byte of tightness bits (which of
-//                its predecessors reach its H, E and F values), laid out step-major
-//                per stripe, so a step's 64 bytes are one coalesced store.
+//  k_tb_dp:      four pairs per wave, one per 16-lane row ("group").  Lane l of a
+//                group owns columns 64 s + 4 l + 1 .. + 4 of stripe s and walks the
+//                rectangle's rows along anti-diagonals (row t - l + 1 at step t);
+//                the left neighbour's H and E arrive through a one-lane DPP shift
+//                inside the row, the stripe's left boundary column through memory,
+//                the substitution scores of the lane's four columns from a per-stripe
+//                profile in LDS (one 8-byte read per step).  Each cell stores one
+//                byte of tightness bits (which of its predecessors reach its H, E and
+//                F values); a lane's four bytes of a step are one dword, a group's
+//                sixteen dwords one coalesced 64-byte store.
 //  k_tb_walk:    one lane per pair: walks the bits back from the rectangle's end
 //                corner, choosing the smallest op that an optimal alignment can
 //                continue with, and writes run-length ops, last run first.
@@ -43,12 +46,6 @@ enum : uint32_t {
 
 enum : uint32_t { kOpM = 0, kOpI = 1, kOpD = 2 }; // BAM CIGAR op codes
 
-// value of the lane one below (wave_shr:1); lane 0 receives 0 (bound_ctrl)
-__device__ __forceinline__ int32_t from_left(int32_t v)
-{
-    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
-}
-
 struct PairRect {
     int32_t m, n;    // rectangle rows (ref side) and columns (query side)
     bool swap;       // rows of the forward pass are the query (|ref| < |query|)
@@ -69,69 +66,152 @@ __device__ __forceinline__ PairRect pair_rect(const TbArgs& a, int64_t p)
     return r;
 }
 
+constexpr int kTbLanes = 16;              // lanes per pair
+constexpr int kTbK = 4;                   // columns per lane
+constexpr int kTbCols = kTbStripeColumns; // columns per stripe (kTbLanes x kTbK)
+static_assert(kTbCols == kTbLanes * kTbK, "stripe width");
+constexpr int kTbLag = kTbLanes - 1;      // steps by which the last lane trails the first
+constexpr int kTbGroups = 64 / kTbLanes;  // pairs per wave (workgroup)
+constexpr int kTbBlock = 64;              // steps between refills of a group's row ring
+constexpr int kTbRing = 128;              // row codes per group: a block's rows and the 15 before
+
+// Bytes of a pair's tightness bits: per stripe, (rows + 15) steps x 16 lanes x 4 columns.
 __host__ __device__ __forceinline__ int64_t tb_dir_bytes(int32_t m, int32_t n)
 {
-    const int64_t stripes = (n + 63) / 64;
-    return (stripes * (int64_t)(m + 63) * 64 + 7) & ~(int64_t)7;
+    const int64_t stripes = (n + kTbCols - 1) / kTbCols;
+    return (stripes * (int64_t)(m + kTbLag) * kTbCols + 7) & ~(int64_t)7;
 }
 
-constexpr int kTbWaves = 4;    // pairs (waves) per workgroup of k_tb_dp
-constexpr int kTbBlock = 256;  // steps between refills of a wave's row ring
-constexpr int kTbRing = 512;   // row codes per wave: a block's rows and the 63 before
-
-__global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count)
+// LDS of k_tb_dp: byte -> code table, alpha^2 scores, then per group the profile
+// ((alpha + 1) rows x 16 lanes x 4 int16), the row ring (codes as profile row
+// offsets) and the block's boundary entries.
+struct TbLds {
+    int32_t sub_off, group_off, group_bytes, ring_off, bnd_off, total;
+};
+__host__ __device__ __forceinline__ TbLds tb_lds(int32_t alpha)
 {
-    __shared__ uint8_t lut[256];
-    __shared__ int16_t sub[kMaxAlpha * kMaxAlpha];
-    __shared__ uint8_t ring_all[kTbWaves][kTbRing];
-    __shared__ uint64_t bndbuf_all[kTbWaves][kTbBlock];
-    for (int x = threadIdx.x; x < 256; x += 64 * kTbWaves) lut[x] = a.lut[x];
-    for (int x = threadIdx.x; x < a.alpha * a.alpha; x += 64 * kTbWaves) sub[x] = a.sub[x];
-    __syncthreads();
-    // one wave per pair from here on: no workgroup barriers below
+    TbLds L;
+    L.sub_off = 256;
+    L.group_off = (L.sub_off + 2 * alpha * alpha + 15) & ~15;
+    const int32_t prof = (alpha + 1) * kTbLanes * kTbK * 2;
+    L.ring_off = prof;
+    L.bnd_off = (L.ring_off + 2 * kTbRing + 15) & ~15;
+    L.group_bytes = L.bnd_off + 8 * kTbBlock;
+    L.total = L.group_off + kTbGroups * L.group_bytes;
+    return L;
+}
 
-    // the wave index in a scalar register: the pair's sizes and addresses stay scalar
-    const int l = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t k = (int64_t)blockIdx.x * kTbWaves + wv;
-    if (k >= count) return;
-    const int64_t p = a.order[a.p0 + k];
-    if (a.out[p] <= 0) {
-        if (l == 0) a.path_score[p] = 0;
-        return;
+// value of the lane one below inside the 16-lane row (row_shr:1); lane 0 of the
+// row receives 0 (bound_ctrl; every group's first lane takes the boundary instead)
+__device__ __forceinline__ int32_t from_left16(int32_t v)
+{
+    return __builtin_amdgcn_mov_dpp(v, 0x111, 0xF, 0xF, true);
+}
+
+__global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
+{
+    extern __shared__ __align__(16) uint8_t tb_smem[];
+    const TbLds L = tb_lds(a.alpha);
+    uint8_t* lut = tb_smem;
+    int16_t* sub = (int16_t*)(tb_smem + L.sub_off);
+    for (int x = threadIdx.x; x < 256; x += 64) lut[x] = a.lut[x];
+    for (int x = threadIdx.x; x < a.alpha * a.alpha; x += 64) sub[x] = a.sub[x];
+    __syncthreads();
+
+    const int lane = threadIdx.x, g = lane / kTbLanes, l = lane % kTbLanes;
+    uint8_t* gb = tb_smem + L.group_off + g * L.group_bytes;
+    int16_t* prof = (int16_t*)gb;
+    uint16_t* ring = (uint16_t*)(gb + L.ring_off);
+    uint64_t* bndbuf = (uint64_t*)(gb + L.bnd_off);
+    const int32_t alpha = a.alpha, go = a.gap_open, ge = a.gap_extend;
+
+    // this group's pair (none: an idle group that computes nothing it stores)
+    const int64_t k = (int64_t)blockIdx.x * kTbGroups + g;
+    int64_t p = -1;
+    if (k < count) {
+        p = a.order[a.p0 + k];
+        if (a.out[p] <= 0) {
+            if (l == 0) a.path_score[p] = 0;
+            p = -1;
+        }
     }
-    uint8_t* ring = ring_all[wv];
-    uint64_t* bndbuf = bndbuf_all[wv];
-    const PairRect r = pair_rect(a, p);
-    const int32_t m = r.m, n = r.n, alpha = a.alpha;
-    const int32_t go = a.gap_open, ge = a.gap_extend;
-    uint8_t* dir = a.dir + a.dir_off[k];
-    uint64_t* bnd = (uint64_t*)(dir + tb_dir_bytes(m, n)); // boundary column, rows 1..m
-    const int stripes = (n + 63) / 64;
+    PairRect r{0, 0, false, nullptr, nullptr};
+    if (p >= 0) r = pair_rect(a, p);
+    const int32_t m = r.m, n = r.n;
+    const int stripes_g = (p >= 0) ? (n + kTbCols - 1) / kTbCols : 0;
+    uint8_t* dir = (p >= 0) ? a.dir + a.dir_off[k] : nullptr;
+    uint64_t* bnd = (p >= 0) ? (uint64_t*)(dir + tb_dir_bytes(m, n)) : nullptr; // boundary column, rows 1..m
+    // wave-uniform extents: the largest stripe and step counts of the four groups
+    int stripes = stripes_g, rows = stripes_g ? m : 0;
+#pragma unroll
+    for (int o = kTbLanes; o < 64; o <<= 1) {
+        stripes = max(stripes, __shfl_xor(stripes, o));
+        rows = max(rows, __shfl_xor(rows, o));
+    }
+    stripes = __builtin_amdgcn_readfirstlane(stripes);
+    rows = __builtin_amdgcn_readfirstlane(rows);
+    const int32_t steps = rows + kTbLag;
+
     for (int s = 0; s < stripes; ++s) {
-        const int32_t c = 64 * s + l + 1;
-        const bool vc = c <= n;
-        const int cb = vc ? lut[r.B[c - 1]] : 0;
-        int32_t Hup = go + (c - 1) * ge, Fup = kTbNeg; // row 0
-        int32_t Hout = 0, Eout = kTbNeg, Hdiag = 0;
-        // byte offset of this lane's cell at step t: dcol + 64 t (a pair's region
-        // is below 4 GiB)
-        const uint32_t dcol = (uint32_t)((int64_t)s * (m + 63) * 64) + (uint32_t)l;
-        for (int32_t t0 = 0; t0 < m + 63; t0 += kTbBlock) {
-            // rows [t0, t0 + 256) enter the ring as codes (rows t0 - 63 .. t0 - 1
-            // stay from the previous block), with the left boundary of those rows;
-            // lane l reads row t - l at step t, lane 0 the boundary of row t + 1
-            for (int q = 0; q < kTbBlock / 64; ++q) {
-                const int32_t x = t0 + q * 64 + l;
-                if (x < m) ring[x & (kTbRing - 1)] = lut[r.A[x]];
-                if (s > 0 && x + 1 <= m)
-                    bndbuf[q * 64 + l] = __hip_atomic_load(bnd + x + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool live = s < stripes_g;
+        const int32_t c0 = kTbCols * s + kTbK * l + 1; // the lane's first column (1-based)
+        // profile of the stripe: row a, lane l, column k = s(code a, column c0 + k)
+        // in the pair's orientation (row code a indexes the rectangle's rows)
+        {
+            int cb[kTbK];
+#pragma unroll
+            for (int q = 0; q < kTbK; ++q) {
+                const int32_t c = c0 + q;
+                cb[q] = (live && c <= n) ? lut[r.B[c - 1]] : -1;
+                cb[q] = cb[q] >= alpha ? -1 : cb[q];
+            }
+            for (int row = 0; row <= alpha; ++row) {
+                uint32_t w[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    int32_t v2[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int c = cb[2 * h + e];
+                        v2[e] = (row < alpha && c >= 0) ? (r.swap ? sub[c * alpha + row] : sub[row * alpha + c]) : 0;
+                    }
+                    w[h] = (uint32_t)(uint16_t)v2[0] | ((uint32_t)(uint16_t)v2[1] << 16);
+                }
+                *(uint2*)(prof + (row * kTbLanes + l) * kTbK) = make_uint2(w[0], w[1]);
+            }
+        }
+        // row 0 of the rectangle: H(0, c) = open + (c - 1) extend, F = -inf
+        int32_t Hup[kTbK], Fup[kTbK];
+#pragma unroll
+        for (int q = 0; q < kTbK; ++q) {
+            Hup[q] = go + (c0 + q - 1) * ge;
+            Fup[q] = kTbNeg;
+        }
+        int32_t Hout = 0, Eout = kTbNeg;
+        int32_t Hdiag = 0; // H(row - 1, c0 - 1), set from the left input of the previous step
+        const uint32_t dbase = (uint32_t)((int64_t)s * (m + kTbLag) * kTbCols) + (uint32_t)(4 * l);
+        for (int32_t t0 = 0; t0 < steps; t0 += kTbBlock) {
+            // rows t0 + 1 .. t0 + 64 enter the ring as profile offsets (the 15 rows
+            // before stay from the previous block), and lane 0's boundary entries
+            // of those rows (stripe s - 1 is complete: this wave wrote them)
+#pragma unroll
+            for (int q = 0; q < kTbBlock / kTbLanes; ++q) {
+                const int32_t x = t0 + q * kTbLanes + l; // row x + 1
+                int code = alpha;
+                if (live && x < m) {
+                    code = lut[r.A[x]];
+                    code = code > alpha ? alpha : code;
+                }
+                ring[x & (kTbRing - 1)] = (uint16_t)(code * kTbLanes * kTbK * 2);
+                if (live && s > 0 && x < m)
+                    bndbuf[q * kTbLanes + l] = __hip_atomic_load(bnd + x + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __builtin_amdgcn_s_waitcnt(0); // this wave's ring writes land before its reads
-            const int32_t t1 = min(t0 + kTbBlock, m + 63);
+            const int32_t t1 = min(t0 + kTbBlock, steps);
             for (int32_t t = t0; t < t1; ++t) {
-                const int32_t i = t - l + 1;
-                // left boundary of lane 0's row t + 1 (a uniform branch and a
-                // broadcast read; rows past the rectangle read unused entries)
+                const int32_t i = t - l + 1; // row of this lane
+                // left inputs: lane l - 1's last column of the previous step; lane 0
+                // takes the left boundary of row t + 1
                 int32_t bH, bE;
                 if (s == 0) {
                     bH = go + t * ge; // H(t + 1, 0): a gap from the corner
@@ -141,31 +221,52 @@ __global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count
                     bH = (int32_t)(uint32_t)v;
                     bE = (int32_t)(uint32_t)(v >> 32);
                 }
-                // the shifts run in every lane (a DPP read from a lane outside the
+                // (the shifts run in every lane: a DPP read from a lane outside the
                 // exec mask would return the fill value)
-                const int32_t sH = from_left(Hout), sE = from_left(Eout);
+                const int32_t sH = from_left16(Hout), sE = from_left16(Eout);
                 const int32_t Hl = l == 0 ? bH : sH;
                 const int32_t El = l == 0 ? bE : sE;
-                if (i >= 1 && i <= m && vc) {
-                    const int32_t diag = i == 1 ? (c == 1 ? 0 : go + (c - 2) * ge) : Hdiag;
-                    const int code = ring[(t - l) & (kTbRing - 1)];
-                    const int32_t sv = r.swap ? sub[cb * alpha + code] : sub[code * alpha + cb];
-                    const int32_t E = max(El + ge, Hl + go);
-                    const int32_t F = max(Fup + ge, Hup + go);
-                    const int32_t D = diag + sv;
-                    const int32_t H = max(D, max(E, F));
-                    const uint32_t bits = (H == D ? kHD : 0u) | (H == F ? kHF : 0u) |
-                                          (F == Fup + ge ? kFExt : 0u) | (F == Hup + go ? kFOpen : 0u) |
-                                          (E == El + ge ? kEExt : 0u) | (E == Hl + go ? kEOpen : 0u);
-                    dir[dcol + (uint32_t)t * 64u] = (uint8_t)bits;
-                    Hup = H;
-                    Fup = F;
-                    Hout = H;
-                    Eout = E;
-                    if (l == 63 && s + 1 < stripes) bnd[i] = (uint64_t)(uint32_t)H | ((uint64_t)(uint32_t)E << 32);
-                    if (i == m && c == n) a.path_score[p] = H;
-                }
+                const int32_t Hd = Hdiag; // H(i - 1, c0 - 1)
                 Hdiag = Hl;
+                // a lane's state changes only on its rows of the rectangle; its outputs
+                // on other rows reach only its right neighbour's rows outside it
+                if (live && i >= 1 && i <= m) {
+                    // the lane's four scores of row i
+                    const uint2 pw = *(const uint2*)((const uint8_t*)prof + ring[(i - 1) & (kTbRing - 1)] + l * kTbK * 2);
+                    const int32_t sc[kTbK] = {(int32_t)(int16_t)pw.x, (int32_t)pw.x >> 16, (int32_t)(int16_t)pw.y,
+                                              (int32_t)pw.y >> 16};
+                    int32_t diag = i == 1 ? (c0 == 1 ? 0 : go + (c0 - 2) * ge) : Hd;
+                    int32_t Hleft = Hl, Eleft = El;
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int q = 0; q < kTbK; ++q) {
+                        const int32_t E = max(Eleft + ge, Hleft + go);
+                        const int32_t F = max(Fup[q] + ge, Hup[q] + go);
+                        const int32_t D = diag + sc[q];
+                        const int32_t H = max(D, max(E, F));
+                        const uint32_t bits = (H == D ? kHD : 0u) | (H == F ? kHF : 0u) |
+                                              (F == Fup[q] + ge ? kFExt : 0u) | (F == Hup[q] + go ? kFOpen : 0u) |
+                                              (E == Eleft + ge ? kEExt : 0u) | (E == Hleft + go ? kEOpen : 0u);
+                        word |= bits << (8 * q);
+                        diag = Hup[q]; // H(i - 1, column q): the next column's diagonal
+                        Hup[q] = H;
+                        Fup[q] = F;
+                        Hleft = H;
+                        Eleft = E;
+                    }
+                    Hout = Hleft;
+                    Eout = Eleft;
+                    *(uint32_t*)(dir + dbase + (uint32_t)t * kTbCols) = word;
+                    if (l == kTbLanes - 1 && s + 1 < stripes_g)
+                        bnd[i] = (uint64_t)(uint32_t)Hout | ((uint64_t)(uint32_t)Eout << 32);
+                    if (i == m && n >= c0 && n < c0 + kTbK) {
+                        int32_t hn = 0;
+#pragma unroll
+                        for (int q = 0; q < kTbK; ++q)
+                            if (c0 + q == n) hn = Hup[q];
+                        a.path_score[p] = hn;
+                    }
+                }
             }
         }
         // the boundary column is in memory before the next stripe's agent-scope
@@ -174,10 +275,11 @@ __global__ __launch_bounds__(64 * kTbWaves) void k_tb_dp(TbArgs a, int64_t count
     }
 }
 
+// bits of cell (i, j): stripe s, lane l, column q of the lane; step i - 1 + l
 __device__ __forceinline__ uint32_t tb_bits(const uint8_t* dir, int32_t m, int32_t i, int32_t j)
 {
-    const int32_t s = (j - 1) >> 6, l = (j - 1) & 63;
-    return dir[((int64_t)s * (m + 63) + (i + l - 1)) * 64 + l];
+    const int32_t s = (j - 1) / kTbCols, c = (j - 1) % kTbCols, l = c / kTbK, q = c % kTbK;
+    return dir[((int64_t)s * (m + kTbLag) + (i - 1 + l)) * kTbCols + kTbK * l + q];
 }
 
 __global__ __launch_bounds__(64) void k_tb_walk(TbArgs a, int64_t count)
@@ -251,15 +353,15 @@ __global__ __launch_bounds__(256) void k_tb_compact(const uint32_t* runs, const 
 int64_t tb_region_bytes(int32_t m, int32_t n)
 {
     if (m <= 0 || n <= 0) return 0;
-    const int64_t bnd = n > 64 ? 8 * ((int64_t)m + 1) : 0;
+    const int64_t bnd = n > kTbCols ? 8 * ((int64_t)m + 1) : 0;
     return tb_dir_bytes(m, n) + bnd;
 }
 
 hipError_t tb_launch_dp(const TbArgs& a, int64_t count, hipStream_t stream)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tb_dp, dim3((unsigned)((count + kTbWaves - 1) / kTbWaves)), dim3(64 * kTbWaves), 0, stream, a,
-                       count);
+    hipLaunchKernelGGL(k_tb_dp, dim3((unsigned)((count + kTbGroups - 1) / kTbGroups)), dim3(64),
+                       (size_t)tb_lds(a.alpha).total, stream, a, count);
     return hipGetLastError();
 }
 
