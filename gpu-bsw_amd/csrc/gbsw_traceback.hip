@@ -16,10 +16,10 @@
 //                the left neighbour's H and E arrive through a one-lane DPP shift
 //                inside the row, the stripe's left boundary column through memory,
 //                the substitution scores of the lane's four columns from a per-stripe
-//                profile in LDS (one 8-byte read per step).  Each cell stores one
-//                byte of tightness bits (which of its predecessors reach its H, E and
-//                F values); a lane's four bytes of a step are one dword, a group's
-//                sixteen dwords one coalesced 64-byte store.
+//                profile in LDS (one 8-byte read per step).  Each cell gets six
+//                tightness bits (which of its predecessors reach its H, E and F
+//                values); a lane's four cells of a step share one dword, a group's
+//                sixteen dwords are one coalesced 64-byte store.
 //  k_tb_walk:    one lane per pair: walks the bits back from the rectangle's end
 //                corner, choosing the smallest op that an optimal alignment can
 //                continue with, and writes run-length ops, last run first.
@@ -35,13 +35,15 @@ namespace {
 
 constexpr int32_t kTbNeg = -(1 << 29);
 
+// Tightness bits of a cell (six; a lane's four cells of a step share one dword,
+// column q of the lane in bits 6 (3 - q) .. 6 (3 - q) + 5)
 enum : uint32_t {
     kHD = 1u,      // H == diagonal + s
     kHF = 2u,      // H == F (H == E when neither of these holds)
-    kFExt = 8u,    // F == F(up) + extend
-    kFOpen = 16u,  // F == H(up) + open
-    kEExt = 32u,   // E == E(left) + extend
-    kEOpen = 64u,  // E == H(left) + open
+    kFExt = 4u,    // F == F(up) + extend
+    kFOpen = 8u,   // F == H(up) + open
+    kEExt = 16u,   // E == E(left) + extend
+    kEOpen = 32u,  // E == H(left) + open
 };
 
 enum : uint32_t { kOpM = 0, kOpI = 1, kOpD = 2 }; // BAM CIGAR op codes
@@ -106,6 +108,14 @@ __host__ __device__ __forceinline__ TbLds tb_lds(int32_t alpha)
 __device__ __forceinline__ int32_t from_left16(int32_t v)
 {
     return __builtin_amdgcn_mov_dpp(v, 0x111, 0xF, 0xF, true);
+}
+
+// 2 acc + (x == y): the compare's VCC is the carry-in of a doubling add (two
+// instructions per bit; the compiler's own form is a compare, a select and a merge)
+__device__ __forceinline__ uint32_t shift_in_eq(uint32_t acc, int32_t x, int32_t y)
+{
+    asm("v_cmp_eq_u32 vcc, %1, %2\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc" : "+v"(acc) : "v"(x), "v"(y) : "vcc");
+    return acc;
 }
 
 __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
@@ -244,10 +254,13 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
                         const int32_t F = max(Fup[q] + ge, Hup[q] + go);
                         const int32_t D = diag + sc[q];
                         const int32_t H = max(D, max(E, F));
-                        const uint32_t bits = (H == D ? kHD : 0u) | (H == F ? kHF : 0u) |
-                                              (F == Fup[q] + ge ? kFExt : 0u) | (F == Hup[q] + go ? kFOpen : 0u) |
-                                              (E == Eleft + ge ? kEExt : 0u) | (E == Hleft + go ? kEOpen : 0u);
-                        word |= bits << (8 * q);
+                        // the six bits, most significant first
+                        word = shift_in_eq(word, E, Hleft + go);  // kEOpen
+                        word = shift_in_eq(word, E, Eleft + ge);  // kEExt
+                        word = shift_in_eq(word, F, Hup[q] + go); // kFOpen
+                        word = shift_in_eq(word, F, Fup[q] + ge); // kFExt
+                        word = shift_in_eq(word, H, F);           // kHF
+                        word = shift_in_eq(word, H, D);           // kHD
                         diag = Hup[q]; // H(i - 1, column q): the next column's diagonal
                         Hup[q] = H;
                         Fup[q] = F;
@@ -279,7 +292,8 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
 __device__ __forceinline__ uint32_t tb_bits(const uint8_t* dir, int32_t m, int32_t i, int32_t j)
 {
     const int32_t s = (j - 1) / kTbCols, c = (j - 1) % kTbCols, l = c / kTbK, q = c % kTbK;
-    return dir[((int64_t)s * (m + kTbLag) + (i - 1 + l)) * kTbCols + kTbK * l + q];
+    const uint32_t w = *(const uint32_t*)(dir + ((int64_t)s * (m + kTbLag) + (i - 1 + l)) * kTbCols + kTbK * l);
+    return (w >> (6 * (kTbK - 1 - q))) & 63u;
 }
 
 __global__ __launch_bounds__(64) void k_tb_walk(TbArgs a, int64_t count)
