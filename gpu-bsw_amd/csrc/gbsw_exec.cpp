@@ -504,10 +504,13 @@ bool plan_is_split(const BucketPlan& b)
 // 32-bit words of a split bucket's bookkeeping: keys (2 x count u64), progress
 // words (2 slots per workgroup, u64), done counters (2 x quads), release epochs
 // (one per slot) and the placement table (kXcds).
+// Split bookkeeping of one bucket in 32-bit words, a multiple of 64 (256 bytes):
+// the next bucket's 64-bit keys and progress words stay 8-byte aligned.
 size_t split_words_of(const Bucket& bk, const BucketPlan& b)
 {
     const size_t nquads = (size_t)((bk.count + kPairsPerWave - 1) / kPairsPerWave);
-    return 4 * (size_t)bk.count + 2 * nquads + 3 * (size_t)(2 * b.grid) + kXcds;
+    const size_t words = 5 * (size_t)bk.count + 2 * nquads + 3 * (size_t)(2 * b.grid) + kXcds;
+    return (words + 63) & ~(size_t)63;
 }
 
 // Kernels, grids and scratch of every bucket (host only; reserves the buffers).
@@ -557,8 +560,8 @@ int plan_batched(Exec& ex, const Job& job, int max_sub, bool p8, bool f16p8, boo
             b.grid = grid_for(ex, b.kind, b.K, b.lds, nquads);
         }
     }
-    // boundary rows and split bookkeeping (keys: 2 x count u64, done: 2 x quads
-    // u32, prog: slots u64, rel: slots u32) in 32-bit words
+    // boundary rows and split bookkeeping (keys: 2 x count u64, prog: slots u64,
+    // done: 2 x quads u32, rel: slots u32, xcc, gate: count u32) in 32-bit words
     size_t bnd_words = 0, split_words = 0;
     auto sizes = [&]() {
         bnd_words = split_words = 0;
@@ -670,6 +673,9 @@ int record_batched(Exec& ex, const Job& job, KernelArgs a, int max_sub, const st
             a.split.done = w + 4 * bk.count + 2 * nslots;
             a.split.rel = a.split.done + 2 * nquads;
             a.split.xcc = a.split.rel + nslots;
+            a.split.gate = a.split.xcc + kXcds;
+            if (((uintptr_t)a.split.keys | (uintptr_t)a.split.prog) & 7)
+                return fail(GBSW_E_INTERNAL, "split bookkeeping of a bucket is not 8-byte aligned");
             a.split.strips = b.strips;
             a.split.slots = (int32_t)(nslots / kXcds);
         }

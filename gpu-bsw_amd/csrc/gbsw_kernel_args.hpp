@@ -51,9 +51,12 @@ struct SplitArgs {
     uint64_t* prog;   // per boundary slot: epoch << 32 | boundary rows ready
     uint32_t* rel;    // per boundary slot: epochs released
     uint32_t* xcc;    // kXcds words: 1 + the hardware XCC id seen by counter x's workgroups (0: none yet)
+    uint32_t* gate;   // order_n words by work slot: last reverse strip's decision for the pair
+                      // (s + 1: strip s keeps it, kGateDropped: dropped from the rest)
     int32_t strips;   // strip items per quad and pass
     int32_t slots;    // boundary slots per XCD
 };
+constexpr uint32_t kGateDropped = 0xFFFFFFFFu;
 constexpr int kXcds = 8;                 // workgroup w is expected on XCD w % 8
 constexpr uint64_t kSpinTicks = 200000000ull; // 2 s of the 100 MHz s_memrealtime clock
 

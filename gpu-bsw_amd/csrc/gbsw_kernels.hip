@@ -462,6 +462,27 @@ __device__ __forceinline__ void wait_at_least(const T* w, T target, int32_t* sta
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
+// Per 16-lane group: wait until the group's word *w >= target (no wait where w
+// is null) and return it (target where w is null).  Bounded like wait_at_least.
+__device__ __forceinline__ uint32_t wait_group_u32(const uint32_t* w, uint32_t target, int32_t* status)
+{
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    uint32_t v = target;
+    for (;;) {
+        if (w) v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__ballot(v < target) == 0) break;
+        int32_t stop = 0;
+        if (threadIdx.x == 0) stop = __hip_atomic_load(status + kStatusTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__builtin_amdgcn_readfirstlane(stop) != 0) break;
+        if (__builtin_amdgcn_s_memrealtime() - t0 > kSpinTicks) {
+            if (threadIdx.x == 0) atomicOr(status + kStatusTimeout, 1);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return v;
+}
 __device__ __forceinline__ void wait_word(const uint64_t* w, uint64_t target, int32_t* status)
 {
     wait_at_least<uint64_t>(w, target, status);
@@ -1278,11 +1299,22 @@ __global__ __launch_bounds__(64) void k_split(KernelArgs a)
             const uint64_t fk = p >= 0 ? __hip_atomic_load(fkey + ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             f = forward_result(a, p, fk, s == 0);
             t = reverse_task(a, p, f, column_bound(a, sm, p, f));
-            // strips past the first that reached the target for every pair of the
-            // quad hold larger columns only: nothing left to find
-            if (p >= 0 && t.nr > 0 && t.nc > 0 &&
-                __hip_atomic_load(rkey + ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
-                t.nr = 0;
+            // A pair whose target an earlier strip reached is dropped from the
+            // rest of its strips: they hold larger columns only.  The drop must be
+            // monotone along the strips (a strip reads the pair's boundary rows,
+            // and waits for the progress word, of the strip before), and two
+            // strips read the winner key at different times, so strip s > 0 waits
+            // for strip s - 1's decision in gate[ws], drops the pair if that
+            // strip did, and publishes its own decision for strip s + 1.
+            uint32_t* gw = p >= 0 ? sp.gate + ws : nullptr;
+            const uint32_t prev = wait_group_u32(s > 0 ? gw : nullptr, (uint32_t)s, a.status);
+            bool keep = t.nr > 0 && t.nc > 0 && prev != kGateDropped;
+            if (keep && s > 0 && __hip_atomic_load(rkey + ws, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                keep = false;
+            if (!keep) t.nr = 0;
+            if (gw && l == 0)
+                __hip_atomic_store(gw, keep ? (uint32_t)s + 1 : kGateDropped, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         // strips of the quad (wave maximum)
         int st = (t.nr > 0 && t.nc > 0) ? (t.nc + kVirtPerPair * K - 1) / (kVirtPerPair * K) : 0;

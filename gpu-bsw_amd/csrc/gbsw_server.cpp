@@ -376,10 +376,11 @@ int gbsw_server::init_slot(Slot& s)
     if (!rc) rc = alloc_into(s.ex.bnd, (size_t)cfg.boundary_bytes_per_slot, device_bytes);
     if (!rc) rc = alloc_into(s.ex.redo, 4 * P, device_bytes);
     if (!rc) rc = alloc_into(s.ex.queue, 4 * kXcds * kMaxBuckets, device_bytes);
-    // split-form bookkeeping: keys and counters of every pair, and the progress
-    // words of up to two boundary slots per workgroup of a full grid per bucket
-    const size_t split_words = 4 * P + 2 * (P / kPairsPerWave + kMaxBuckets) +
-                               6 * (size_t)kMaxBuckets * 16 * s.ex.num_cu + (size_t)kXcds * kMaxBuckets;
+    // split-form bookkeeping: keys, strip gates and counters of every pair, the
+    // progress words of up to two boundary slots per workgroup of a full grid
+    // per bucket, and each bucket's padding to 256 bytes
+    const size_t split_words = 5 * P + 2 * (P / kPairsPerWave + kMaxBuckets) +
+                               6 * (size_t)kMaxBuckets * 16 * s.ex.num_cu + (size_t)(kXcds + 64) * kMaxBuckets;
     if (!rc) rc = alloc_into(s.ex.split_buf, 4 * split_words, device_bytes);
     s.ex.fixed = true;
     if (rc) return rc;
