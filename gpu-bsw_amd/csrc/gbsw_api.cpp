@@ -497,12 +497,14 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     std::vector<uint16_t> cls((size_t)n);
     std::vector<int64_t> cls_count(kTbClasses, 0);
     cap_off[0] = 0;
+    int64_t max_span = 0; // largest rows + columns of a rectangle
     for (int64_t p = 0; p < n; ++p) {
         const bool on = res[(size_t)p] > 0;
         const int32_t m = on ? res[(size_t)(2 * n + p)] - res[(size_t)(n + p)] : 0;
         const int32_t w = on ? res[(size_t)(4 * n + p)] - res[(size_t)(3 * n + p)] : 0;
         if (on && (m <= 0 || w <= 0)) return fail(GBSW_E_INTERNAL, "a pair with a positive score has an empty rectangle");
         if ((int64_t)(m + w) * big >= ((int64_t)1 << 28)) return fail(GBSW_E_RANGE, "path scores may exceed the int32 cells' range");
+        max_span = std::max<int64_t>(max_span, (int64_t)m + w);
         cap_off[(size_t)p + 1] = cap_off[(size_t)p] + m + w;
         region[(size_t)p] = tb_region_bytes(m, w);
         const int32_t stripes = on ? std::min(127, (w + kTbStripeColumns - 1) / kTbStripeColumns) : 0;
@@ -519,12 +521,23 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
         for (int c = kTbClasses - 1; c >= 0; --c) { start[(size_t)c] = acc; acc += cls_count[(size_t)c]; }
         for (int64_t p = 0; p < n; ++p) order[(size_t)start[cls[(size_t)p]]++] = p;
     }
-    // launches of at most kTbScratch bytes of scratch (or one pair), in that order
+    // int16 cells when every value stays inside +-32,000: a path value lies within
+    // (rows + columns + 2) x the largest |score| of zero, and each sum formed from
+    // one adds one score more
+    const bool cells16 = (max_span + 4) * big < 32000;
+    // launches in that order: one ends past kTbScratch bytes of scratch once it
+    // holds enough waves to fill the GPU (six per SIMD; the largest pairs come
+    // first, and a launch of few of them would run latency-bound), and in any case
+    // before kTbScratchMax (or at one pair)
+    constexpr int64_t kTbScratchMax = (int64_t)12 << 30;
+    const int64_t fill_pairs = (int64_t)ctx->num_cu * 4 * 6 * (cells16 ? 8 : 4);
     std::vector<std::pair<int64_t, int64_t>> launches; // (first position, count)
     int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
     for (int64_t x = 0; x < n; ++x) {
         const int64_t bytes = region[(size_t)order[(size_t)x]];
-        if (chunk_bytes + bytes > kTbScratch && x > chunk_first) {
+        const bool full = chunk_bytes + bytes > kTbScratchMax ||
+                          (chunk_bytes + bytes > kTbScratch && x - chunk_first >= fill_pairs);
+        if (full && x > chunk_first) {
             launches.emplace_back(chunk_first, x - chunk_first);
             max_chunk = std::max(max_chunk, chunk_bytes);
             chunk_first = x;
@@ -574,32 +587,40 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     a.nruns = d_nruns;
     a.path_score = d_score;
     a.order = d_order;
+    a.cells16 = cells16 ? 1 : 0;
     Exec& ex = ctx->ex;
-    // DP kernels on the context stream, walks on an auxiliary one (created here
-    // when the context runs its buckets on one stream); launch c's walk waits for
-    // its DP, and launch c + 2's DP (same scratch half) for that walk
-    if (int rc = ex.ensure_streams(2)) return rc;
+    // Two pipelines, the launches at even and at odd positions, each with its own
+    // scratch half, DP stream and walk stream (the context stream and three
+    // auxiliary ones): launch c's walk waits for its DP, and launch c + 2's DP
+    // (same half) for that walk.  A launch lasts as long as its longest wave (the
+    // largest pairs come first), so the other pipeline's launches fill the GPU
+    // while one runs out its tail.
+    if (int rc = ex.ensure_streams(4)) return rc;
     GBSW_HIP_TRY(hipEventRecord(ex.ev_start, st));
-    hipStream_t sw = ex.aux[0];
-    while (b->tb_events.size() < 2 * launches.size()) {
+    const hipStream_t dp_st[2] = {st, ex.aux[1]}, walk_st[2] = {ex.aux[0], ex.aux[2]};
+    const size_t nev = 2 * launches.size() + 1; // DP c done, walk c done; the tables uploaded
+    while (b->tb_events.size() < nev) {
         hipEvent_t e;
         GBSW_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->tb_events.push_back(e);
     }
-    GBSW_HIP_TRY(hipEventRecord(b->tb_events[0], st));
-    GBSW_HIP_TRY(hipStreamWaitEvent(sw, b->tb_events[0], 0)); // the tables are uploaded
+    hipEvent_t uploaded = b->tb_events[nev - 1];
+    GBSW_HIP_TRY(hipEventRecord(uploaded, st));
+    for (int x = 0; x < 3; ++x) GBSW_HIP_TRY(hipStreamWaitEvent(ex.aux[x], uploaded, 0));
     for (size_t c = 0; c < launches.size(); ++c) {
+        const size_t h = c & 1;
         a.p0 = launches[c].first;
         a.dir_off = d_dir_off + launches[c].first;
-        a.dir = b->tb_dir.as<uint8_t>() + (c & 1) * half;
-        if (c >= 2) GBSW_HIP_TRY(hipStreamWaitEvent(st, b->tb_events[2 * (c - 2) + 1], 0));
-        GBSW_HIP_TRY(tb_launch_dp(a, launches[c].second, st));
-        GBSW_HIP_TRY(hipEventRecord(b->tb_events[2 * c], st));
-        GBSW_HIP_TRY(hipStreamWaitEvent(sw, b->tb_events[2 * c], 0));
-        GBSW_HIP_TRY(tb_launch_walk(a, launches[c].second, sw));
-        GBSW_HIP_TRY(hipEventRecord(b->tb_events[2 * c + 1], sw));
+        a.dir = b->tb_dir.as<uint8_t>() + h * half;
+        if (c >= 2) GBSW_HIP_TRY(hipStreamWaitEvent(dp_st[h], b->tb_events[2 * (c - 2) + 1], 0));
+        GBSW_HIP_TRY(tb_launch_dp(a, launches[c].second, dp_st[h]));
+        GBSW_HIP_TRY(hipEventRecord(b->tb_events[2 * c], dp_st[h]));
+        GBSW_HIP_TRY(hipStreamWaitEvent(walk_st[h], b->tb_events[2 * c], 0));
+        GBSW_HIP_TRY(tb_launch_walk(a, launches[c].second, walk_st[h]));
+        GBSW_HIP_TRY(hipEventRecord(b->tb_events[2 * c + 1], walk_st[h]));
     }
-    GBSW_HIP_TRY(hipStreamWaitEvent(st, b->tb_events[2 * (launches.size() - 1) + 1], 0));
+    for (size_t c = launches.size() >= 2 ? launches.size() - 2 : 0; c < launches.size(); ++c)
+        GBSW_HIP_TRY(hipStreamWaitEvent(st, b->tb_events[2 * c + 1], 0));
     GBSW_HIP_TRY(hipEventRecord(ex.ev_stop, st)); // device_ms: the path kernels
     GBSW_HIP_TRY(hipMemcpyAsync(nruns, d_nruns, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
     if (path_score)

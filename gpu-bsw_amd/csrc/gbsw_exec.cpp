@@ -909,8 +909,11 @@ int complete_job(Exec& ex, const Job& job, const gbsw_scoring* sc, double* devic
         ex.launches[i].ms = t;
     }
     if (status[kStatusMisplaced]) return fail(GBSW_E_INTERNAL, "split-form placement check failed after an unsplit rerun");
-    if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
+    // a timed-out wait lets every later wait return at once, after which passes
+    // may read boundary rows that are not there: report the timeout, not what
+    // follows from it
     if (status[kStatusTimeout]) return fail(GBSW_E_INTERNAL, "a split-form wait timed out");
+    if (status[kStatusRevMiss]) return fail(GBSW_E_INTERNAL, "reverse pass did not reach the forward score");
     return GBSW_OK;
 }
 

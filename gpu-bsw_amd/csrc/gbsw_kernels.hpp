@@ -75,6 +75,7 @@ struct TbArgs {
     uint32_t* runs;         // run-length ops, last run first
     int32_t* nruns;         // per pair: runs written
     int32_t* path_score;    // per pair: score of the path
+    int32_t cells16;        // 1: every path value fits int16 (k_tb_dp16), 0: int32 cells (k_tb_dp)
 };
 // Columns of one stripe of the traceback DP.
 constexpr int kTbStripeColumns = 64;

@@ -10,6 +10,8 @@
 // op sequence, read from its end, is lexicographically smallest with M < D < I.
 //
 // Three kernels per call:
+//  k_tb_dp16 / k_tb_dp (int16 / int32 cells; the host takes the int16 form when
+//                every path value of the batch fits it):
 //  k_tb_dp:      four pairs per wave, one per 16-lane row ("group").  Lane l of a
 //                group owns columns 64 s + 4 l + 1 .. + 4 of stripe s and walks the
 //                rectangle's rows along anti-diagonals (row t - l + 1 at step t);
@@ -20,6 +22,9 @@
 //                tightness bits (which of its predecessors reach its H, E and F
 //                values); a lane's four cells of a step share one dword, a group's
 //                sixteen dwords are one coalesced 64-byte store.
+//  k_tb_dp16:    the same with two pairs per lane, one in each 16-bit half of every
+//                state register (eight pairs per wave): each v_pk_* op advances
+//                two cells, and a bit costs three packed ops for two cells.
 //  k_tb_walk:    one lane per pair: walks the bits back from the rectangle's end
 //                corner, choosing the smallest op that an optimal alignment can
 //                continue with, and writes run-length ops, last run first.
@@ -27,6 +32,7 @@
 //                place.
 #include <hip/hip_runtime.h>
 
+#include "gbsw_device.hpp"
 #include "gbsw_kernels.hpp"
 
 namespace gbsw {
@@ -35,8 +41,9 @@ namespace {
 
 constexpr int32_t kTbNeg = -(1 << 29);
 
-// Tightness bits of a cell (six; a lane's four cells of a step share one dword,
-// column q of the lane in bits 6 (3 - q) .. 6 (3 - q) + 5)
+// Tightness bits of a cell (six; a lane's four cells of a step share one dword:
+// columns 0 and 1 of the lane in bits 6..11 and 0..5, columns 2 and 3 in bits
+// 22..27 and 16..21, so that either half of the dword is one 16-bit accumulator)
 enum : uint32_t {
     kHD = 1u,      // H == diagonal + s
     kHF = 2u,      // H == F (H == E when neither of these holds)
@@ -247,7 +254,7 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
                                               (int32_t)pw.y >> 16};
                     int32_t diag = i == 1 ? (c0 == 1 ? 0 : go + (c0 - 2) * ge) : Hd;
                     int32_t Hleft = Hl, Eleft = El;
-                    uint32_t word = 0;
+                    uint32_t word = 0, word01 = 0;
 #pragma unroll
                     for (int q = 0; q < kTbK; ++q) {
                         const int32_t E = max(Eleft + ge, Hleft + go);
@@ -266,10 +273,14 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
                         Fup[q] = F;
                         Hleft = H;
                         Eleft = E;
+                        if (q == 1) {
+                            word01 = word;
+                            word = 0;
+                        }
                     }
                     Hout = Hleft;
                     Eout = Eleft;
-                    *(uint32_t*)(dir + dbase + (uint32_t)t * kTbCols) = word;
+                    *(uint32_t*)(dir + dbase + (uint32_t)t * kTbCols) = word01 | (word << 16);
                     if (l == kTbLanes - 1 && s + 1 < stripes_g)
                         bnd[i] = (uint64_t)(uint32_t)Hout | ((uint64_t)(uint32_t)Eout << 32);
                     if (i == m && n >= c0 && n < c0 + kTbK) {
@@ -288,12 +299,272 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
     }
 }
 
+// ---- k_tb_dp16: two pairs per lane, one per 16-bit half
+
+constexpr int kTb16Pairs = 2 * kTbGroups; // pairs per wave
+
+// LDS of k_tb_dp16: byte -> code table, alpha^2 scores, then per group the two
+// pairs' profiles, the row ring (both pairs' profile offsets in one dword) and the
+// block's boundary entries (H and E of both pairs, packed).
+struct TbLds16 {
+    int32_t sub_off, group_off, prof_bytes, ring_off, bnd_off, group_bytes, total;
+};
+__host__ __device__ __forceinline__ TbLds16 tb_lds16(int32_t alpha)
+{
+    TbLds16 L;
+    L.sub_off = 256;
+    L.group_off = (L.sub_off + 2 * alpha * alpha + 15) & ~15;
+    L.prof_bytes = (alpha + 1) * kTbLanes * kTbK * 2;
+    L.ring_off = 2 * L.prof_bytes;
+    L.bnd_off = (L.ring_off + 4 * kTbRing + 15) & ~15;
+    L.group_bytes = L.bnd_off + 8 * kTbBlock;
+    L.total = L.group_off + kTbGroups * L.group_bytes;
+    return L;
+}
+
+__device__ __forceinline__ uint32_t splat16(int32_t v)
+{
+    const uint32_t h = (uint32_t)(uint16_t)(int16_t)v;
+    return h | (h << 16);
+}
+__device__ __forceinline__ int32_t half16(uint32_t x, int h)
+{
+    return (int32_t)(int16_t)(uint16_t)(h ? x >> 16 : x & 0xFFFFu);
+}
+// (acc << 1) | (x == y) in each half, for x >= y: v_pk_sub_u16 gives x - y,
+// v_pk_sub_u16 clamp gives 1 - (x - y) floored at 0 (1 exactly when equal), and
+// v_lshl_or_b32 shifts it in (an accumulator half holds at most 12 bits, so
+// nothing crosses into the other half)
+__device__ __forceinline__ uint32_t shift_in_eq16(uint32_t acc, uint32_t x, uint32_t y)
+{
+    const u16x2 d = as_u(x) - as_u(y);
+    const uint32_t z = raw(__builtin_elementwise_sub_sat(as_u(0x00010001u), d));
+    return (acc << 1) | z;
+}
+
+// Cells are int16 with saturating adds: the host takes this kernel when
+// (rows + columns + 4) x the largest |score| of every pair stays below 32,000, so
+// every path value and every sum formed from one stays inside the int16 range
+// and only the -inf entries (-32768) saturate; they equal no cell value, as the
+// int32 kernel's -2^29 does not.  Bits and boundary columns are those of
+// k_tb_dp, in the same layout.
+__global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
+{
+    extern __shared__ __align__(16) uint8_t tb_smem[];
+    const TbLds16 L = tb_lds16(a.alpha);
+    uint8_t* lut = tb_smem;
+    int16_t* sub = (int16_t*)(tb_smem + L.sub_off);
+    for (int x = threadIdx.x; x < 256; x += 64) lut[x] = a.lut[x];
+    for (int x = threadIdx.x; x < a.alpha * a.alpha; x += 64) sub[x] = a.sub[x];
+    __syncthreads();
+
+    const int lane = threadIdx.x, g = lane / kTbLanes, l = lane % kTbLanes;
+    uint8_t* gb = tb_smem + L.group_off + g * L.group_bytes;
+    uint32_t* ring = (uint32_t*)(gb + L.ring_off);
+    uint2* bndbuf = (uint2*)(gb + L.bnd_off);
+    const int32_t alpha = a.alpha, go = a.gap_open, ge = a.gap_extend;
+
+    // the group's two pairs, h = 0 in the low halves (none: an idle half whose
+    // cells are computed and never stored)
+    int64_t p[2];
+    PairRect r[2];
+    int32_t m[2], n[2];
+    int stripes_h[2];
+    uint8_t* dir[2];
+    uint64_t* bnd[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int64_t k = (int64_t)blockIdx.x * kTb16Pairs + 2 * g + h;
+        p[h] = -1;
+        if (k < count) {
+            p[h] = a.order[a.p0 + k];
+            if (a.out[p[h]] <= 0) {
+                if (l == 0) a.path_score[p[h]] = 0;
+                p[h] = -1;
+            }
+        }
+        r[h] = PairRect{0, 0, false, nullptr, nullptr};
+        if (p[h] >= 0) r[h] = pair_rect(a, p[h]);
+        m[h] = r[h].m;
+        n[h] = r[h].n;
+        stripes_h[h] = (p[h] >= 0) ? (n[h] + kTbCols - 1) / kTbCols : 0;
+        dir[h] = (p[h] >= 0) ? a.dir + a.dir_off[k] : nullptr;
+        bnd[h] = (p[h] >= 0) ? (uint64_t*)(dir[h] + tb_dir_bytes(m[h], n[h])) : nullptr;
+    }
+    // wave-uniform extents: the largest stripe and row counts of the eight pairs
+    int stripes = max(stripes_h[0], stripes_h[1]);
+    int rows = max(stripes_h[0] ? m[0] : 0, stripes_h[1] ? m[1] : 0);
+#pragma unroll
+    for (int o = kTbLanes; o < 64; o <<= 1) {
+        stripes = max(stripes, __shfl_xor(stripes, o));
+        rows = max(rows, __shfl_xor(rows, o));
+    }
+    stripes = __builtin_amdgcn_readfirstlane(stripes);
+    rows = __builtin_amdgcn_readfirstlane(rows);
+    const int32_t steps = rows + kTbLag;
+    const uint32_t GO2 = splat16(go), GE2 = splat16(ge), NEG2 = 0x80008000u;
+
+    for (int s = 0; s < stripes; ++s) {
+        const bool live[2] = {s < stripes_h[0], s < stripes_h[1]};
+        const int32_t mlive = max(live[0] ? m[0] : 0, live[1] ? m[1] : 0);
+        const int32_t c0 = kTbCols * s + kTbK * l + 1; // the lane's first column (1-based)
+        // profiles of the stripe (one per pair, as in k_tb_dp)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            int16_t* prof = (int16_t*)(gb + h * L.prof_bytes);
+            int cb[kTbK];
+#pragma unroll
+            for (int q = 0; q < kTbK; ++q) {
+                const int32_t c = c0 + q;
+                cb[q] = (live[h] && c <= n[h]) ? lut[r[h].B[c - 1]] : -1;
+                cb[q] = cb[q] >= alpha ? -1 : cb[q];
+            }
+            for (int row = 0; row <= alpha; ++row) {
+                uint32_t w[2];
+#pragma unroll
+                for (int e2 = 0; e2 < 2; ++e2) {
+                    int32_t v2[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int c = cb[2 * e2 + e];
+                        v2[e] = (row < alpha && c >= 0) ? (r[h].swap ? sub[c * alpha + row] : sub[row * alpha + c]) : 0;
+                    }
+                    w[e2] = (uint32_t)(uint16_t)v2[0] | ((uint32_t)(uint16_t)v2[1] << 16);
+                }
+                *(uint2*)(prof + (row * kTbLanes + l) * kTbK) = make_uint2(w[0], w[1]);
+            }
+        }
+        // row 0 of the rectangle: H(0, c) = open + (c - 1) extend, F = -inf
+        uint32_t Hup[kTbK], Fup[kTbK];
+#pragma unroll
+        for (int q = 0; q < kTbK; ++q) {
+            Hup[q] = splat16(go + (c0 + q - 1) * ge);
+            Fup[q] = NEG2;
+        }
+        uint32_t Hout = 0, Eout = NEG2;
+        uint32_t Hdiag = 0; // H(row - 1, c0 - 1), set from the left input of the previous step
+        const uint32_t diag1 = splat16(c0 == 1 ? 0 : go + (c0 - 2) * ge); // H(0, c0 - 1)
+        uint32_t dbase[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            dbase[h] = (uint32_t)((int64_t)s * (m[h] + kTbLag) * kTbCols) + (uint32_t)(4 * l);
+        for (int32_t t0 = 0; t0 < steps; t0 += kTbBlock) {
+            // rows t0 + 1 .. t0 + 64 of both pairs enter the ring (profile offsets)
+            // and, after the first stripe, lane 0's boundary entries of those rows
+#pragma unroll
+            for (int q = 0; q < kTbBlock / kTbLanes; ++q) {
+                const int32_t x = t0 + q * kTbLanes + l; // row x + 1
+                uint32_t off[2];
+                uint64_t v[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    int code = alpha;
+                    v[h] = 0;
+                    if (live[h] && x < m[h]) {
+                        code = lut[r[h].A[x]];
+                        code = code > alpha ? alpha : code;
+                        if (s > 0)
+                            v[h] = __hip_atomic_load(bnd[h] + x + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    off[h] = (uint32_t)(code * kTbLanes * kTbK * 2);
+                }
+                ring[x & (kTbRing - 1)] = off[0] | (off[1] << 16);
+                if (s > 0)
+                    bndbuf[q * kTbLanes + l] =
+                        make_uint2(__builtin_amdgcn_perm((uint32_t)v[1], (uint32_t)v[0], 0x05040100u),
+                                   __builtin_amdgcn_perm((uint32_t)(v[1] >> 32), (uint32_t)(v[0] >> 32), 0x05040100u));
+            }
+            __builtin_amdgcn_s_waitcnt(0); // this wave's ring writes land before its reads
+            const int32_t t1 = min(t0 + kTbBlock, steps);
+            for (int32_t t = t0; t < t1; ++t) {
+                const int32_t i = t - l + 1; // row of this lane
+                uint32_t bH, bE;
+                if (s == 0) {
+                    bH = splat16(go + t * ge); // H(t + 1, 0): a gap from the corner
+                    bE = NEG2;
+                } else {
+                    const uint2 v = bndbuf[t - t0];
+                    bH = v.x;
+                    bE = v.y;
+                }
+                const uint32_t sH = (uint32_t)from_left16((int32_t)Hout), sE = (uint32_t)from_left16((int32_t)Eout);
+                const uint32_t Hl = l == 0 ? bH : sH;
+                const uint32_t El = l == 0 ? bE : sE;
+                const uint32_t Hd = Hdiag; // H(i - 1, c0 - 1)
+                Hdiag = Hl;
+                if (i >= 1 && i <= mlive) {
+                    // the lane's four scores of row i, pair 0 low and pair 1 high
+                    const uint32_t rw = ring[(i - 1) & (kTbRing - 1)];
+                    const uint2 pa = *(const uint2*)(gb + (rw & 0xFFFFu) + l * kTbK * 2);
+                    const uint2 pb = *(const uint2*)(gb + L.prof_bytes + (rw >> 16) + l * kTbK * 2);
+                    const uint32_t sc[kTbK] = {__builtin_amdgcn_perm(pb.x, pa.x, 0x05040100u),
+                                               __builtin_amdgcn_perm(pb.x, pa.x, 0x07060302u),
+                                               __builtin_amdgcn_perm(pb.y, pa.y, 0x05040100u),
+                                               __builtin_amdgcn_perm(pb.y, pa.y, 0x07060302u)};
+                    uint32_t diag = i == 1 ? diag1 : Hd;
+                    uint32_t Hleft = Hl, Eleft = El;
+                    uint32_t word = 0, word01 = 0;
+#pragma unroll
+                    for (int q = 0; q < kTbK; ++q) {
+                        const uint32_t eo = pk_add_sat(Hleft, GO2), ee = pk_add_sat(Eleft, GE2);
+                        const uint32_t fo = pk_add_sat(Hup[q], GO2), fe = pk_add_sat(Fup[q], GE2);
+                        const uint32_t E = pk_max(eo, ee);
+                        const uint32_t F = pk_max(fo, fe);
+                        const uint32_t D = pk_add_sat(diag, sc[q]);
+                        const uint32_t H = pk_max(D, pk_max(E, F));
+                        // the six bits, most significant first
+                        word = shift_in_eq16(word, E, eo); // kEOpen
+                        word = shift_in_eq16(word, E, ee); // kEExt
+                        word = shift_in_eq16(word, F, fo); // kFOpen
+                        word = shift_in_eq16(word, F, fe); // kFExt
+                        word = shift_in_eq16(word, H, F);  // kHF
+                        word = shift_in_eq16(word, H, D);  // kHD
+                        diag = Hup[q];
+                        Hup[q] = H;
+                        Fup[q] = F;
+                        Hleft = H;
+                        Eleft = E;
+                        if (q == 1) {
+                            word01 = word;
+                            word = 0;
+                        }
+                    }
+                    Hout = Hleft;
+                    Eout = Eleft;
+                    // each pair's dword: its halves of the two accumulators
+                    const uint32_t dw[2] = {__builtin_amdgcn_perm(word, word01, 0x05040100u),
+                                            __builtin_amdgcn_perm(word, word01, 0x07060302u)};
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (live[h] && i <= m[h]) {
+                            *(uint32_t*)(dir[h] + dbase[h] + (uint32_t)t * kTbCols) = dw[h];
+                            if (l == kTbLanes - 1 && s + 1 < stripes_h[h])
+                                bnd[h][i] = (uint64_t)(uint32_t)half16(Hout, h) |
+                                            ((uint64_t)(uint32_t)half16(Eout, h) << 32);
+                            if (i == m[h] && n[h] >= c0 && n[h] < c0 + kTbK) {
+                                int32_t hn = 0;
+#pragma unroll
+                                for (int q = 0; q < kTbK; ++q)
+                                    if (c0 + q == n[h]) hn = half16(Hup[q], h);
+                                a.path_score[p[h]] = hn;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // the boundary columns are in memory before the next stripe's agent-scope
+        // loads read them (one wave: its own stores, complete)
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+}
+
 // bits of cell (i, j): stripe s, lane l, column q of the lane; step i - 1 + l
 __device__ __forceinline__ uint32_t tb_bits(const uint8_t* dir, int32_t m, int32_t i, int32_t j)
 {
     const int32_t s = (j - 1) / kTbCols, c = (j - 1) % kTbCols, l = c / kTbK, q = c % kTbK;
     const uint32_t w = *(const uint32_t*)(dir + ((int64_t)s * (m + kTbLag) + (i - 1 + l)) * kTbCols + kTbK * l);
-    return (w >> (6 * (kTbK - 1 - q))) & 63u;
+    return (w >> (16 * (q >> 1) + 6 * (1 - (q & 1)))) & 63u;
 }
 
 __global__ __launch_bounds__(64) void k_tb_walk(TbArgs a, int64_t count)
@@ -374,8 +645,13 @@ int64_t tb_region_bytes(int32_t m, int32_t n)
 hipError_t tb_launch_dp(const TbArgs& a, int64_t count, hipStream_t stream)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tb_dp, dim3((unsigned)((count + kTbGroups - 1) / kTbGroups)), dim3(64),
-                       (size_t)tb_lds(a.alpha).total, stream, a, count);
+    if (a.cells16) {
+        hipLaunchKernelGGL(k_tb_dp16, dim3((unsigned)((count + kTb16Pairs - 1) / kTb16Pairs)), dim3(64),
+                           (size_t)tb_lds16(a.alpha).total, stream, a, count);
+    } else {
+        hipLaunchKernelGGL(k_tb_dp, dim3((unsigned)((count + kTbGroups - 1) / kTbGroups)), dim3(64),
+                           (size_t)tb_lds(a.alpha).total, stream, a, count);
+    }
     return hipGetLastError();
 }
 
