@@ -535,8 +535,9 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
     for (int64_t x = 0; x < n; ++x) {
         const int64_t bytes = region[(size_t)order[(size_t)x]];
+        const int64_t held = x - chunk_first;
         const bool full = chunk_bytes + bytes > kTbScratchMax ||
-                          (chunk_bytes + bytes > kTbScratch && x - chunk_first >= fill_pairs);
+                          (held >= fill_pairs && (chunk_bytes + bytes > kTbScratch || held >= 2 * fill_pairs));
         if (full && x > chunk_first) {
             launches.emplace_back(chunk_first, x - chunk_first);
             max_chunk = std::max(max_chunk, chunk_bytes);

@@ -41,9 +41,13 @@ namespace {
 
 constexpr int32_t kTbNeg = -(1 << 29);
 
-// Tightness bits of a cell (six; a lane's four cells of a step share one dword:
-// columns 0 and 1 of the lane in bits 6..11 and 0..5, columns 2 and 3 in bits
-// 22..27 and 16..21, so that either half of the dword is one 16-bit accumulator)
+// Tightness of a cell: six bits, a lane's four cells of a step in one dword
+// (columns 0 and 1 of the lane in bits 6..11 and 0..5, columns 2 and 3 in bits
+// 22..27 and 16..21, so that either half of the dword is one 16-bit accumulator).
+// Stored, most significant first: the E field (2 bits: 0 when only E(left) +
+// extend reaches E, 2 when only H(left) + open does, 1 when both), the F field
+// (likewise, from above), then H == F and H == D; tb_bits turns them into these
+// flags.
 enum : uint32_t {
     kHD = 1u,      // H == diagonal + s
     kHF = 2u,      // H == F (H == E when neither of these holds)
@@ -261,13 +265,11 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
                         const int32_t F = max(Fup[q] + ge, Hup[q] + go);
                         const int32_t D = diag + sc[q];
                         const int32_t H = max(D, max(E, F));
-                        // the six bits, most significant first
-                        word = shift_in_eq(word, E, Hleft + go);  // kEOpen
-                        word = shift_in_eq(word, E, Eleft + ge);  // kEExt
-                        word = shift_in_eq(word, F, Hup[q] + go); // kFOpen
-                        word = shift_in_eq(word, F, Fup[q] + ge); // kFExt
-                        word = shift_in_eq(word, H, F);           // kHF
-                        word = shift_in_eq(word, H, D);           // kHD
+                        // the E and F fields, then the two H bits
+                        word = (word << 2) | (uint32_t)(1 + (E == Hleft + go) - (E == Eleft + ge));
+                        word = (word << 2) | (uint32_t)(1 + (F == Hup[q] + go) - (F == Fup[q] + ge));
+                        word = shift_in_eq(word, H, F); // kHF
+                        word = shift_in_eq(word, H, D); // kHD
                         diag = Hup[q]; // H(i - 1, column q): the next column's diagonal
                         Hup[q] = H;
                         Fup[q] = F;
@@ -331,15 +333,44 @@ __device__ __forceinline__ int32_t half16(uint32_t x, int h)
 {
     return (int32_t)(int16_t)(uint16_t)(h ? x >> 16 : x & 0xFFFFu);
 }
-// (acc << 1) | (x == y) in each half, for x >= y: v_pk_sub_u16 gives x - y,
-// v_pk_sub_u16 clamp gives 1 - (x - y) floored at 0 (1 exactly when equal), and
-// v_lshl_or_b32 shifts it in (an accumulator half holds at most 12 bits, so
-// nothing crosses into the other half)
-__device__ __forceinline__ uint32_t shift_in_eq16(uint32_t acc, uint32_t x, uint32_t y)
+// A cell's six tightness bits shifted into each half of acc (modulo 2^16 per half,
+// v_pk_mad_u16, so a field written as its value - 1 borrows only inside its half;
+// tb_fields_bias adds the ones back), for E = max(eo, ee), F = max(fo, fe) and
+// H = max(D, E, F):
+//   field - 1 of E (of F) = the sign of eo - ee: v_pk_sub_i16 clamp, then
+//     v_pk_min_i16 / v_pk_max_i16 to [-1, 1];
+//   H == F (H == D): v_pk_sub_u16 gives H - F >= 0, v_pk_sub_u16 clamp turns it
+//     into 1 - (H - F) floored at 0, 1 exactly when equal.
+// (Written as asm: the compiler's own form of the clamp was two compares and two
+// selects per half, and of the multiply-add a shift and an add.)
+__device__ __forceinline__ uint32_t tb_cell_bits16(uint32_t acc, uint32_t eo, uint32_t ee, uint32_t fo, uint32_t fe,
+                                                   uint32_t H, uint32_t F, uint32_t D)
 {
-    const u16x2 d = as_u(x) - as_u(y);
-    const uint32_t z = raw(__builtin_elementwise_sub_sat(as_u(0x00010001u), d));
-    return (acc << 1) | z;
+    uint32_t t0, t1;
+    asm("v_pk_sub_i16 %1, %3, %4 clamp\n\t"
+        "v_pk_sub_i16 %2, %5, %6 clamp\n\t"
+        "v_pk_min_i16 %1, %1, 1 op_sel_hi:[1,0]\n\t"
+        "v_pk_min_i16 %2, %2, 1 op_sel_hi:[1,0]\n\t"
+        "v_pk_max_i16 %1, %1, -1\n\t"
+        "v_pk_max_i16 %2, %2, -1\n\t"
+        "v_pk_mad_u16 %0, %0, 4, %1 op_sel_hi:[1,0,1]\n\t"
+        "v_pk_mad_u16 %0, %0, 4, %2 op_sel_hi:[1,0,1]\n\t"
+        "v_pk_sub_u16 %1, %7, %8\n\t"
+        "v_pk_sub_u16 %2, %7, %9\n\t"
+        "v_pk_sub_u16 %1, 1, %1 op_sel_hi:[0,1] clamp\n\t"
+        "v_pk_sub_u16 %2, 1, %2 op_sel_hi:[0,1] clamp\n\t"
+        "v_pk_mad_u16 %0, %0, 2, %1 op_sel_hi:[1,0,1]\n\t"
+        "v_pk_mad_u16 %0, %0, 2, %2 op_sel_hi:[1,0,1]"
+        : "+v"(acc), "=&v"(t0), "=&v"(t1)
+        : "v"(eo), "v"(ee), "v"(fo), "v"(fe), "v"(H), "v"(F), "v"(D));
+    return acc;
+}
+// the ones of the four fields of an accumulator half (two columns: the E and F
+// fields end at bits 4 and 2 of a column's six, the first column six higher)
+constexpr uint32_t kTbFieldOnes = (1u << 10) | (1u << 8) | (1u << 4) | (1u << 2);
+__device__ __forceinline__ uint32_t tb_fields_bias(uint32_t acc)
+{
+    return raw(as_u(acc) + (u16x2){(uint16_t)kTbFieldOnes, (uint16_t)kTbFieldOnes});
 }
 
 // Cells are int16 with saturating adds: the host takes this kernel when
@@ -512,13 +543,8 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                         const uint32_t F = pk_max(fo, fe);
                         const uint32_t D = pk_add_sat(diag, sc[q]);
                         const uint32_t H = pk_max(D, pk_max(E, F));
-                        // the six bits, most significant first
-                        word = shift_in_eq16(word, E, eo); // kEOpen
-                        word = shift_in_eq16(word, E, ee); // kEExt
-                        word = shift_in_eq16(word, F, fo); // kFOpen
-                        word = shift_in_eq16(word, F, fe); // kFExt
-                        word = shift_in_eq16(word, H, F);  // kHF
-                        word = shift_in_eq16(word, H, D);  // kHD
+                        // the E and F fields, then H == F and H == D
+                        word = tb_cell_bits16(word, eo, ee, fo, fe, H, F, D);
                         diag = Hup[q];
                         Hup[q] = H;
                         Fup[q] = F;
@@ -531,6 +557,8 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                     }
                     Hout = Hleft;
                     Eout = Eleft;
+                    word01 = tb_fields_bias(word01);
+                    word = tb_fields_bias(word);
                     // each pair's dword: its halves of the two accumulators
                     const uint32_t dw[2] = {__builtin_amdgcn_perm(word, word01, 0x05040100u),
                                             __builtin_amdgcn_perm(word, word01, 0x07060302u)};
@@ -564,7 +592,10 @@ __device__ __forceinline__ uint32_t tb_bits(const uint8_t* dir, int32_t m, int32
 {
     const int32_t s = (j - 1) / kTbCols, c = (j - 1) % kTbCols, l = c / kTbK, q = c % kTbK;
     const uint32_t w = *(const uint32_t*)(dir + ((int64_t)s * (m + kTbLag) + (i - 1 + l)) * kTbCols + kTbK * l);
-    return (w >> (16 * (q >> 1) + 6 * (1 - (q & 1)))) & 63u;
+    const uint32_t b = (w >> (16 * (q >> 1) + 6 * (1 - (q & 1)))) & 63u;
+    const uint32_t ef = b >> 4, ff = (b >> 2) & 3u;
+    return (b & (kHD | kHF)) | (ef >= 1 ? kEOpen : 0u) | (ef <= 1 ? kEExt : 0u) | (ff >= 1 ? kFOpen : 0u) |
+           (ff <= 1 ? kFExt : 0u);
 }
 
 __global__ __launch_bounds__(64) void k_tb_walk(TbArgs a, int64_t count)
