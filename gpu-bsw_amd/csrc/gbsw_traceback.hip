@@ -373,6 +373,10 @@ __device__ __forceinline__ uint32_t tb_fields_bias(uint32_t acc)
     return raw(as_u(acc) + (u16x2){(uint16_t)kTbFieldOnes, (uint16_t)kTbFieldOnes});
 }
 
+// The state holds H + open (Ho) in place of H: both open candidates of a cell are
+// then the neighbours' values as they are, and the profile holds s - open so that
+// the diagonal Ho(i-1, j-1) + s - open is D itself.  (H + open is a value of the
+// same range.)
 // Cells are int16 with saturating adds: the host takes this kernel when
 // (rows + columns + 4) x the largest |score| of every pair stays below 32,000, so
 // every path value and every sum formed from one stays inside the int16 range
@@ -458,7 +462,7 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
 #pragma unroll
                     for (int e = 0; e < 2; ++e) {
                         const int c = cb[2 * e2 + e];
-                        v2[e] = (row < alpha && c >= 0) ? (r[h].swap ? sub[c * alpha + row] : sub[row * alpha + c]) : 0;
+                        v2[e] = ((row < alpha && c >= 0) ? (r[h].swap ? sub[c * alpha + row] : sub[row * alpha + c]) : 0) - go;
                     }
                     w[e2] = (uint32_t)(uint16_t)v2[0] | ((uint32_t)(uint16_t)v2[1] << 16);
                 }
@@ -466,15 +470,20 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
             }
         }
         // row 0 of the rectangle: H(0, c) = open + (c - 1) extend, F = -inf
-        uint32_t Hup[kTbK], Fup[kTbK];
+        uint32_t Hup[kTbK], Fup[kTbK]; // Ho and F of the row above
 #pragma unroll
         for (int q = 0; q < kTbK; ++q) {
-            Hup[q] = splat16(go + (c0 + q - 1) * ge);
+            Hup[q] = splat16(2 * go + (c0 + q - 1) * ge);
             Fup[q] = NEG2;
         }
-        uint32_t Hout = 0, Eout = NEG2;
-        uint32_t Hdiag = 0; // H(row - 1, c0 - 1), set from the left input of the previous step
-        const uint32_t diag1 = splat16(c0 == 1 ? 0 : go + (c0 - 2) * ge); // H(0, c0 - 1)
+        uint32_t Hout = splat16(go), Eout = NEG2; // Ho and E of the lane's last column
+        uint32_t Hdiag = 0; // Ho(row - 1, c0 - 1), set from the left input of the previous step
+        const uint32_t diag1 = splat16((c0 == 1 ? 0 : go + (c0 - 2) * ge) + go); // Ho(0, c0 - 1)
+        // the step at which this lane meets a pair's end corner (row m, column n), -1: none
+        int32_t tpath[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            tpath[h] = (live[h] && n[h] >= c0 && n[h] < c0 + kTbK) ? m[h] + l - 1 : -1;
         uint32_t dbase[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -511,7 +520,7 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                 const int32_t i = t - l + 1; // row of this lane
                 uint32_t bH, bE;
                 if (s == 0) {
-                    bH = splat16(go + t * ge); // H(t + 1, 0): a gap from the corner
+                    bH = splat16(2 * go + t * ge); // Ho(t + 1, 0): a gap from the corner
                     bE = NEG2;
                 } else {
                     const uint2 v = bndbuf[t - t0];
@@ -521,7 +530,7 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                 const uint32_t sH = (uint32_t)from_left16((int32_t)Hout), sE = (uint32_t)from_left16((int32_t)Eout);
                 const uint32_t Hl = l == 0 ? bH : sH;
                 const uint32_t El = l == 0 ? bE : sE;
-                const uint32_t Hd = Hdiag; // H(i - 1, c0 - 1)
+                const uint32_t Hd = Hdiag; // Ho(i - 1, c0 - 1)
                 Hdiag = Hl;
                 if (i >= 1 && i <= mlive) {
                     // the lane's four scores of row i, pair 0 low and pair 1 high
@@ -537,18 +546,19 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                     uint32_t word = 0, word01 = 0;
 #pragma unroll
                     for (int q = 0; q < kTbK; ++q) {
-                        const uint32_t eo = pk_add_sat(Hleft, GO2), ee = pk_add_sat(Eleft, GE2);
-                        const uint32_t fo = pk_add_sat(Hup[q], GO2), fe = pk_add_sat(Fup[q], GE2);
+                        const uint32_t eo = Hleft, ee = pk_add_sat(Eleft, GE2);
+                        const uint32_t fo = Hup[q], fe = pk_add_sat(Fup[q], GE2);
                         const uint32_t E = pk_max(eo, ee);
                         const uint32_t F = pk_max(fo, fe);
                         const uint32_t D = pk_add_sat(diag, sc[q]);
                         const uint32_t H = pk_max(D, pk_max(E, F));
                         // the E and F fields, then H == F and H == D
                         word = tb_cell_bits16(word, eo, ee, fo, fe, H, F, D);
+                        const uint32_t Ho = pk_add_sat(H, GO2);
                         diag = Hup[q];
-                        Hup[q] = H;
+                        Hup[q] = Ho;
                         Fup[q] = F;
-                        Hleft = H;
+                        Hleft = Ho;
                         Eleft = E;
                         if (q == 1) {
                             word01 = word;
@@ -569,11 +579,11 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                             if (l == kTbLanes - 1 && s + 1 < stripes_h[h])
                                 bnd[h][i] = (uint64_t)(uint32_t)half16(Hout, h) |
                                             ((uint64_t)(uint32_t)half16(Eout, h) << 32);
-                            if (i == m[h] && n[h] >= c0 && n[h] < c0 + kTbK) {
+                            if (t == tpath[h]) {
                                 int32_t hn = 0;
 #pragma unroll
                                 for (int q = 0; q < kTbK; ++q)
-                                    if (c0 + q == n[h]) hn = half16(Hup[q], h);
+                                    if (c0 + q == n[h]) hn = half16(Hup[q], h) - go;
                                 a.path_score[p[h]] = hn;
                             }
                         }
