@@ -220,7 +220,10 @@ def config_bench(args, only=None):
         d = {"pairs": w.n, "cells": cells, "batched_gcups": round(rb["gcups"], 2),
              "batched_device_gcups": round(rb["device_gcups"], 2), "batched_ms": round(rb["ms_per_step"], 3),
              "int32_redone_pairs": int(sum(L["pairs"] for L in rb["launches"]
-                                           if L["kind"] & 0xFF == gbsw.KIND_BATCHED_WIDE))}
+                                           if L["kind"] & 0xFF == gbsw.KIND_BATCHED_WIDE)),
+             # the whole step against the packed int16 VALU roofline (same op model as the
+             # headline's aggregate fraction)
+             "roofline_frac": round(OPS_PER_CELL * cells / (rb["ms_per_step"] * 1e-3) / 1e12 / PEAK_INT_TOPS_PACKED, 4)}
         if plain:
             rp = time_mode(b, sc, "plain", 1, 2, cells)
             res_p = b.results()
