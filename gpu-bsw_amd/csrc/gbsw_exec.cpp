@@ -491,7 +491,11 @@ double bucket_cost(const BucketPlan& b, const Bucket& bk)
 {
     const int k = b.kind & ~kByteProfile;
     const double f16 = (k == kBatchedF16 || k == kSplitF16) ? 1.0 : 1.15;
-    const double split = (k == kSplit || k == kSplitF16) ? 2.5 : 1.0;
+    // a split bucket's strips wait on each other, and beside the bulk buckets it
+    // runs several times longer than alone (the K = 7 split bucket of the 1M-pair
+    // DNA mix: 1.0 ms alone, 6.6 ms beside them); weights 2.5 / 4 / 6 / 8 / 12
+    // measured 3,658-3,672 / 3,703-3,708 / 3,715-3,729 / 3,717-3,726 / 3,713 GCUPS
+    const double split = (k == kSplit || k == kSplitF16) ? 6.0 : 1.0;
     return (double)bk.cells * (b.K + 2) / b.K * f16 * split;
 }
 
@@ -503,9 +507,9 @@ bool plan_is_split(const BucketPlan& b)
 
 // 32-bit words of a split bucket's bookkeeping: keys (2 x count u64), progress
 // words (2 slots per workgroup, u64), done counters (2 x quads), release epochs
-// (one per slot) and the placement table (kXcds).
-// Split bookkeeping of one bucket in 32-bit words, a multiple of 64 (256 bytes):
-// the next bucket's 64-bit keys and progress words stay 8-byte aligned.
+// (one per slot), the placement table (kXcds) and the strip gates (count), padded
+// to a multiple of 64 (256 bytes) so that the next bucket's 64-bit keys and
+// progress words stay 8-byte aligned.
 size_t split_words_of(const Bucket& bk, const BucketPlan& b)
 {
     const size_t nquads = (size_t)((bk.count + kPairsPerWave - 1) / kPairsPerWave);
