@@ -307,7 +307,7 @@ constexpr int kTb16Pairs = 2 * kTbGroups; // pairs per wave
 
 // LDS of k_tb_dp16: byte -> code table, alpha^2 scores, then per group the two
 // pairs' profiles, the row ring (both pairs' profile offsets in one dword) and the
-// block's boundary entries (H and E of both pairs, packed).
+// block's boundary entries (Ho and E of both pairs, packed).
 struct TbLds16 {
     int32_t sub_off, group_off, prof_bytes, ring_off, bnd_off, group_bytes, total;
 };
@@ -381,8 +381,9 @@ __device__ __forceinline__ uint32_t tb_fields_bias(uint32_t acc)
 // (rows + columns + 4) x the largest |score| of every pair stays below 32,000, so
 // every path value and every sum formed from one stays inside the int16 range
 // and only the -inf entries (-32768) saturate; they equal no cell value, as the
-// int32 kernel's -2^29 does not.  Bits and boundary columns are those of
-// k_tb_dp, in the same layout.
+// int32 kernel's -2^29 does not.  The bits are those of k_tb_dp, in the same
+// layout; the boundary column (read only by this kernel's next stripe) holds Ho
+// and E.
 __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
 {
     extern __shared__ __align__(16) uint8_t tb_smem[];
