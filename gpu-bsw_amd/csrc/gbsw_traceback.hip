@@ -306,10 +306,11 @@ __global__ __launch_bounds__(64) void k_tb_dp(TbArgs a, int64_t count)
 constexpr int kTb16Pairs = 2 * kTbGroups; // pairs per wave
 
 // LDS of k_tb_dp16: byte -> code table, alpha^2 scores, then per group the two
-// pairs' profiles, the row ring (both pairs' profile offsets in one dword) and the
-// block's boundary entries (Ho and E of both pairs, packed).
+// pairs' profiles, the row ring (both pairs' profile offsets in one dword), the
+// block's incoming boundary entries and its outgoing ones (Ho and E of both pairs,
+// packed).
 struct TbLds16 {
-    int32_t sub_off, group_off, prof_bytes, ring_off, bnd_off, group_bytes, total;
+    int32_t sub_off, group_off, prof_bytes, ring_off, bnd_off, bout_off, group_bytes, total;
 };
 __host__ __device__ __forceinline__ TbLds16 tb_lds16(int32_t alpha)
 {
@@ -319,7 +320,8 @@ __host__ __device__ __forceinline__ TbLds16 tb_lds16(int32_t alpha)
     L.prof_bytes = (alpha + 1) * kTbLanes * kTbK * 2;
     L.ring_off = 2 * L.prof_bytes;
     L.bnd_off = (L.ring_off + 4 * kTbRing + 15) & ~15;
-    L.group_bytes = L.bnd_off + 8 * kTbBlock;
+    L.bout_off = L.bnd_off + 8 * kTbBlock;
+    L.group_bytes = L.bout_off + 8 * kTbBlock;
     L.total = L.group_off + kTbGroups * L.group_bytes;
     return L;
 }
@@ -398,6 +400,7 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
     uint8_t* gb = tb_smem + L.group_off + g * L.group_bytes;
     uint32_t* ring = (uint32_t*)(gb + L.ring_off);
     uint2* bndbuf = (uint2*)(gb + L.bnd_off);
+    uint2* bout = (uint2*)(gb + L.bout_off);
     const int32_t alpha = a.alpha, go = a.gap_open, ge = a.gap_extend;
 
     // the group's two pairs, h = 0 in the low halves (none: an idle half whose
@@ -573,13 +576,13 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                     // each pair's dword: its halves of the two accumulators
                     const uint32_t dw[2] = {__builtin_amdgcn_perm(word, word01, 0x05040100u),
                                             __builtin_amdgcn_perm(word, word01, 0x07060302u)};
+                    // the last lane's column goes to the next stripe: staged here,
+                    // stored a block at a time
+                    if (l == kTbLanes - 1) bout[t - t0] = make_uint2(Hout, Eout);
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         if (live[h] && i <= m[h]) {
                             *(uint32_t*)(dir[h] + dbase[h] + (uint32_t)t * kTbCols) = dw[h];
-                            if (l == kTbLanes - 1 && s + 1 < stripes_h[h])
-                                bnd[h][i] = (uint64_t)(uint32_t)half16(Hout, h) |
-                                            ((uint64_t)(uint32_t)half16(Eout, h) << 32);
                             if (t == tpath[h]) {
                                 int32_t hn = 0;
 #pragma unroll
@@ -587,6 +590,22 @@ __global__ __launch_bounds__(64) void k_tb_dp16(TbArgs a, int64_t count)
                                     if (c0 + q == n[h]) hn = half16(Hup[q], h) - go;
                                 a.path_score[p[h]] = hn;
                             }
+                        }
+                    }
+                }
+            }
+            // the block's outgoing boundary entries: step t0 + e holds row t0 + e - 14
+            // of the last lane (written by this group's lane 15; the same wave, so
+            // its LDS writes have landed once the wait returns)
+            __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (s + 1 < stripes_h[h]) {
+                    for (int e = l; e < t1 - t0; e += kTbLanes) {
+                        const int32_t row = t0 + e - (kTbLanes - 2);
+                        if (row >= 1 && row <= m[h]) {
+                            const uint2 v = bout[e];
+                            bnd[h][row] = (uint64_t)(uint32_t)half16(v.x, h) | ((uint64_t)(uint32_t)half16(v.y, h) << 32);
                         }
                     }
                 }
