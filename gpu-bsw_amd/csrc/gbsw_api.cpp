@@ -525,19 +525,28 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     // (rows + columns + 2) x the largest |score| of zero, and each sum formed from
     // one adds one score more
     const bool cells16 = (max_span + 4) * big < 32000;
-    // launches in that order: one ends past kTbScratch bytes of scratch once it
-    // holds enough waves to fill the GPU (six per SIMD; the largest pairs come
-    // first, and a launch of few of them would run latency-bound), and in any case
-    // before kTbScratchMax (or at one pair)
-    constexpr int64_t kTbScratchMax = (int64_t)12 << 30;
+    // launches in that order: one ends past kTbScratch bytes of scratch (scratch_soft)
+    // once it holds enough waves to fill the GPU (six per SIMD; the largest pairs
+    // come first, and a launch of few of them would run latency-bound), at twice
+    // that many pairs, and in any case before scratch_max (or at one pair): 12 GiB,
+    // or a third of the free device memory when less (the scratch of two launches
+    // is held at once)
+    size_t free_bytes = 0, total_bytes = 0;
+    if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        free_bytes = (size_t)24 << 30;
+    }
+    const int64_t scratch_max =
+        std::max<int64_t>((int64_t)256 << 20, std::min<int64_t>((int64_t)12 << 30, (int64_t)(free_bytes / 3)));
+    const int64_t scratch_soft = std::min(kTbScratch, scratch_max);
     const int64_t fill_pairs = (int64_t)ctx->num_cu * 4 * 6 * (cells16 ? 8 : 4);
     std::vector<std::pair<int64_t, int64_t>> launches; // (first position, count)
     int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
     for (int64_t x = 0; x < n; ++x) {
         const int64_t bytes = region[(size_t)order[(size_t)x]];
         const int64_t held = x - chunk_first;
-        const bool full = chunk_bytes + bytes > kTbScratchMax ||
-                          (held >= fill_pairs && (chunk_bytes + bytes > kTbScratch || held >= 2 * fill_pairs));
+        const bool full = chunk_bytes + bytes > scratch_max ||
+                          (held >= fill_pairs && (chunk_bytes + bytes > scratch_soft || held >= 2 * fill_pairs));
         if (full && x > chunk_first) {
             launches.emplace_back(chunk_first, x - chunk_first);
             max_chunk = std::max(max_chunk, chunk_bytes);
