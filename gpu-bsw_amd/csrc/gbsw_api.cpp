@@ -526,11 +526,13 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     // one adds one score more
     const bool cells16 = (max_span + 4) * big < 32000;
     // launches in that order: one ends past kTbScratch bytes of scratch (scratch_soft)
-    // once it holds enough waves to fill the GPU (six per SIMD; the largest pairs
-    // come first, and a launch of few of them would run latency-bound), at twice
-    // that many pairs, and in any case before scratch_max (or at one pair): 12 GiB,
-    // or a third of the free device memory when less (the scratch of two launches
-    // is held at once)
+    // once it holds enough waves to fill the GPU twice over (16 per SIMD, against
+    // the DP's seven resident; the largest pairs come first, and a launch of few of
+    // them runs latency-bound; 6, 8, 10, 12, 14 and 16 measured 1,225-1,231, 1,250-
+    // 1,279, 1,271-1,304, 1,277-1,280, 1,301-1,308 and 1,304-1,310 GCUPS on the
+    // 1M-pair DNA mix), at twice that many pairs, and in any case before
+    // scratch_max (or at one pair): 12 GiB, or a third of the free device memory
+    // when less (the scratch of two launches is held at once)
     size_t free_bytes = 0, total_bytes = 0;
     if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -539,7 +541,7 @@ int gbsw_batch_traceback(gbsw_batch* b, const gbsw_scoring* sc, int64_t capacity
     const int64_t scratch_max =
         std::max<int64_t>((int64_t)256 << 20, std::min<int64_t>((int64_t)12 << 30, (int64_t)(free_bytes / 3)));
     const int64_t scratch_soft = std::min(kTbScratch, scratch_max);
-    const int64_t fill_pairs = (int64_t)ctx->num_cu * 4 * 6 * (cells16 ? 8 : 4);
+    const int64_t fill_pairs = (int64_t)ctx->num_cu * 4 * 16 * (cells16 ? 8 : 4);
     std::vector<std::pair<int64_t, int64_t>> launches; // (first position, count)
     int64_t chunk_first = 0, chunk_bytes = 0, max_chunk = 0;
     for (int64_t x = 0; x < n; ++x) {
