@@ -60,6 +60,47 @@ def test_config2_every_pair_of_1m(ctx):
         b.close()
 
 
+def test_config2_paths_across_launches(ctx):
+    """Alignment paths of all 1,000,000 pairs of the batched DNA mix in one call:
+    its scratch is cut into several launches run as two DP/walk pipelines (the
+    smaller path tests fit one launch).  Every pair's path score equals its local
+    score (the begin and end of every pair of this mix belong to one best local
+    alignment), and every 20th pair's path (50,000 pairs across all launches)
+    equals the oracle's."""
+    w = synth.batched_dna()
+    b = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
+    try:
+        print("case config2 1M paths: align", flush=True)
+        b.align(gbsw_scoring(w), "batched")
+        got = b.results()
+        print("case config2 1M paths: traceback", flush=True)
+        words, off, ps, ms = b.traceback(gbsw_scoring(w))
+    finally:
+        b.close()
+    assert np.array_equal(ps, got["score"]), "a path score differs from its local score"
+    idx = np.arange(0, w.n, 20)
+    ro = w.ref_off.astype(np.int64)
+    qo = w.query_off.astype(np.int64)
+    rl, ql = ro[idx + 1] - ro[idx], qo[idx + 1] - qo[idx]
+    sro = np.zeros(len(idx) + 1, np.uint64)
+    sqo = np.zeros(len(idx) + 1, np.uint64)
+    np.cumsum(rl, out=sro[1:])
+    np.cumsum(ql, out=sqo[1:])
+    srb = np.concatenate([w.ref[ro[k]:ro[k + 1]] for k in idx])
+    sqb = np.concatenate([w.query[qo[k]:qo[k + 1]] for k in idx])
+    res = np.zeros(len(idx), dtype=ol.RESULT_DTYPE)
+    for f in FIELDS:
+        res[f] = got[f][idx]
+    print("case config2 1M paths: oracle on", len(idx), "pairs", flush=True)
+    ew, eoff, eps = ol.traceback_batch(srb, sro, sqb, sqo, oracle_scoring(w), res)
+    assert np.array_equal(eps, ps[idx])
+    for j, k in enumerate(idx):
+        g = words[off[k]:off[k + 1]]
+        e = ew[eoff[j]:eoff[j + 1]]
+        if len(g) != len(e) or not np.array_equal(g, e):
+            raise AssertionError("pair %d: got %s want %s" % (int(k), gbsw.cigar_string(g), ol.cigar_string(e)))
+
+
 def test_config3_sample_of_the_full_set(ctx):
     """The whole 50,000-pair long-protein set is aligned at once (as the bench
     runs it); every 50th pair (1,000 pairs over the whole length range) is
