@@ -166,7 +166,11 @@ int gbsw_batch_autotune(gbsw_batch* batch, const gbsw_scoring* scoring, int32_t 
  *   path_score:    n entries or NULL: the score of each path (equal to the pair's
  *                  score whenever its begin and end belong to one best local
  *                  alignment).
- *   device_ms:     NULL or the duration of the path kernels (hipEvents). */
+ *   device_ms:     NULL or the duration of the path kernels (hipEvents).
+ * Device scratch: about one byte per cell of the pairs' rectangles, processed in
+ * launches of at most 12 GiB (a third of the free device memory when that is
+ * less), two of which are held at once; it stays with the batch until
+ * gbsw_batch_destroy. */
 int gbsw_batch_traceback(gbsw_batch* batch, const gbsw_scoring* scoring, int64_t capacity, uint32_t* cigar,
                          int64_t* cigar_offsets, int32_t* path_score, double* device_ms);
 
