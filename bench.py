@@ -190,9 +190,19 @@ def time_mode(batch, scoring, mode, warmup, steps, cells):
         d = dict(L)
         d["ms"] = float(np.mean([step[i]["ms"] for step in launches]))
         avg.append(d)
-    return {"wall_s": wall, "ms_per_step": 1000.0 * wall / steps, "gcups": cells * steps / wall / 1e9,
+    return {"wall_s": wall, "steps": steps, "ms_per_step": 1000.0 * wall / steps, "gcups": cells * steps / wall / 1e9,
             "device_ms": float(np.mean(device_ms)), "device_gcups": cells / (np.mean(device_ms) / 1000.0) / 1e9,
             "launches": avg, "buckets": st["buckets"]}
+
+
+def steps_for(batch, scoring, mode, floor, target_s=0.3, cap=500):
+    """Timed steps for a config: enough for about target_s of work (a 10k-pair
+    call takes half a millisecond, so a few steps would time host jitter),
+    at least floor; the align it times is one more warm-up."""
+    t = time.perf_counter()
+    batch.align(scoring, mode)
+    dt = time.perf_counter() - t
+    return max(floor, min(cap, int(target_s / max(dt, 1e-6))))
 
 
 def config_bench(args, only=None):
@@ -215,9 +225,11 @@ def config_bench(args, only=None):
         gen_s = time.perf_counter() - t
         sc = gbsw.dna_scoring(*w.scoring) if w.alphabet == gbsw.DNA else gbsw.protein_scoring(BLOSUM62, *w.scoring)
         b = ctx.batch(w.alphabet, w.ref, w.ref_off, w.query, w.query_off)
-        rb = time_mode(b, sc, "batched", 1, 3 if plain else 2, cells)
+        # a profiling run (`only`) keeps one warm-up and two timed aligns
+        steps = 2 if only is not None else steps_for(b, sc, "batched", 3 if plain else 2)
+        rb = time_mode(b, sc, "batched", 1, steps, cells)
         res_b = b.results()
-        d = {"pairs": w.n, "cells": cells, "batched_gcups": round(rb["gcups"], 2),
+        d = {"pairs": w.n, "cells": cells, "batched_steps": rb["steps"], "batched_gcups": round(rb["gcups"], 2),
              "batched_device_gcups": round(rb["device_gcups"], 2), "batched_ms": round(rb["ms_per_step"], 3),
              "int32_redone_pairs": int(sum(L["pairs"] for L in rb["launches"]
                                            if L["kind"] & 0xFF == gbsw.KIND_BATCHED_WIDE)),
@@ -225,9 +237,9 @@ def config_bench(args, only=None):
              # headline's aggregate fraction)
              "roofline_frac": round(OPS_PER_CELL * cells / (rb["ms_per_step"] * 1e-3) / 1e12 / PEAK_INT_TOPS_PACKED, 4)}
         if plain:
-            rp = time_mode(b, sc, "plain", 1, 2, cells)
+            rp = time_mode(b, sc, "plain", 1, steps_for(b, sc, "plain", 2), cells)
             res_p = b.results()
-            d.update({"plain_gcups": round(rp["gcups"], 2), "plain_ms": round(rp["ms_per_step"], 3),
+            d.update({"plain_steps": rp["steps"], "plain_gcups": round(rp["gcups"], 2), "plain_ms": round(rp["ms_per_step"], 3),
                       "batched_over_plain": round(rb["gcups"] / rp["gcups"], 3),
                       "bit_exact_plain_vs_batched": all(np.array_equal(res_b[f], res_p[f]) for f in gbsw.FIELDS)})
         log("%s: %d pairs, %.4g cells (generated in %.1f s): batched %.1f GCUPS%s" %
